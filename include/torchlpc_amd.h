@@ -1,0 +1,119 @@
+/*
+ * torchlpc_amd C ABI: time-varying all-pole (LPC) filter on MI355X (gfx950).
+ *
+ *   y[b,t] = x[b,t] - sum_{i=1..N} A[b,t,i-1] * y[b,t-i],   y[b,-k] = zi[b,k-1]
+ *
+ * Every entry point is a plain C function over device pointers and sizes.
+ * The caller owns all buffers (inputs, outputs and the workspace); the library
+ * never allocates device memory, never synchronises and launches only on the
+ * stream it is given, so every call can be recorded into a hipGraph.
+ *
+ * Layouts (all row-major, contiguous, element type selected by `dtype`):
+ *   x, y, g, dx : [B, T]
+ *   A, dA       : [B, T, N]      (time-varying coefficients)
+ *   a, da       : [B, N]         (time-invariant coefficients, TLPC_TIME_INVARIANT)
+ *   zi, dzi     : [B, N]         zi[b,k] = y[b,-1-k]  (same orientation as the reference)
+ * Ragged (packed) batches describe signal k by offsets instead of B/T:
+ *   x/y/g/dx at x + off_t[k], length len[k];  A/dA at A + off_a[k], [len[k], N_k];
+ *   zi/dzi at zi + off_z[k], N_k values.
+ *
+ * Return value: TLPC_OK (0) or a negative TLPC_E* code; tlpc_error_string()
+ * names it. A failed kernel launch is reported as TLPC_ELAUNCH.
+ */
+#ifndef TORCHLPC_AMD_H
+#define TORCHLPC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types */
+#define TLPC_F32 0
+#define TLPC_F64 1
+#define TLPC_C64 2  /* complex float, interleaved (re, im) */
+#define TLPC_C128 3 /* complex double, interleaved (re, im) */
+
+/* status codes */
+#define TLPC_OK 0
+#define TLPC_EINVAL -1     /* bad size, null pointer, bad flag */
+#define TLPC_EDTYPE -2     /* unsupported element type */
+#define TLPC_EWORKSPACE -3 /* workspace smaller than the matching *_workspace() query */
+#define TLPC_ELAUNCH -4    /* hipGetLastError() reported a launch failure */
+#define TLPC_EORDER -5     /* unsupported filter order */
+
+/* flags for the backward entry points: which gradients to write */
+#define TLPC_NEED_DX 1
+#define TLPC_NEED_DA 2
+#define TLPC_NEED_DZI 4
+/* flag for forward/backward: A is time-invariant, given as a[B, N] */
+#define TLPC_TIME_INVARIANT 8
+/* with TLPC_TIME_INVARIANT in the backward: write the per-sample dA[B,T,N]
+   (the gradient of a time-broadcast A) instead of the reduced da[B,N] */
+#define TLPC_DA_FULL 16
+
+/* strategy for tlpc_lpc_forward / tlpc_lpc_backward (`strategy` argument) */
+#define TLPC_STRATEGY_AUTO 0    /* library heuristic */
+#define TLPC_STRATEGY_CHUNKED 1 /* multi-kernel chunked scan (any B) */
+#define TLPC_STRATEGY_SIGNAL 2  /* one workgroup per signal, fused tile pipeline */
+#define TLPC_STRATEGY_SERIAL 3  /* one thread per signal, no time parallelism */
+
+const char* tlpc_error_string(int code);
+int tlpc_abi_version(void);
+
+/*
+ * Forward filter. Replaces the reference op torchlpc::lpc(x, A, zi)
+ * (torchlpc/csrc/scan_cpu.cpp:143-164 CPU, torchlpc/csrc/cuda/lpc.cu:115-175 CUDA,
+ * schema at scan_cpu.cpp:169). `chunk_len` 0 lets the library choose.
+ */
+size_t tlpc_lpc_forward_workspace(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,
+                                  int32_t strategy, int32_t chunk_len);
+int tlpc_lpc_forward(int dtype, const void* x, const void* A, const void* zi, void* y, int64_t B,
+                     int64_t T, int32_t N, int32_t flags, int32_t strategy, int32_t chunk_len,
+                     void* workspace, size_t workspace_bytes, void* hip_stream);
+
+/*
+ * Fused adjoint. Replaces the backward composition of LPC.backward
+ * (torchlpc/core.py:186-229): one reverse-time pass gives
+ *   dx[t]      = lambda[t] = g[t] - sum_i conj(A[t+i,i]) lambda[t+i]
+ *   dA[t,i-1]  = -lambda[t] * conj(y[t-i])          (y[-k] = zi[k-1])
+ *   dzi[k]     = lambda[-1-k]
+ * With TLPC_TIME_INVARIANT, `dA` receives da[B,N] = -sum_t lambda[t] conj(y[t-i]),
+ * or the per-sample dA[B,T,N] when TLPC_DA_FULL is also set.
+ * Gradients whose TLPC_NEED_* flag is clear may be passed as NULL.
+ */
+size_t tlpc_lpc_backward_workspace(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,
+                                   int32_t strategy, int32_t chunk_len);
+int tlpc_lpc_backward(int dtype, const void* g, const void* A, const void* zi, const void* y,
+                      void* dx, void* dA, void* dzi, int64_t B, int64_t T, int32_t N,
+                      int32_t flags, int32_t strategy, int32_t chunk_len, void* workspace,
+                      size_t workspace_bytes, void* hip_stream);
+
+/*
+ * First-order recurrence h[t] = w[t] * h[t-1] + x[t], h[-1] = init[b] (the order-1
+ * filter with A = -w). Replaces the reference op torchlpc::scan(input, weights,
+ * initials) (torchlpc/csrc/scan_cpu.cpp:28-78,124-141, schema :168;
+ * torchlpc/csrc/cuda/linear_recurrence.cu:232-289). `init` may be NULL (zero).
+ */
+size_t tlpc_scan_workspace(int dtype, int64_t B, int64_t T, int32_t flags, int32_t chunk_len);
+int tlpc_scan(int dtype, const void* x, const void* w, const void* init, void* h, int64_t B,
+              int64_t T, int32_t chunk_len, void* workspace, size_t workspace_bytes,
+              void* hip_stream);
+/*
+ * Its adjoint (reference: Recurrence.backward, torchlpc/recurrence.py:92-126):
+ *   mu[t] = g[t] + conj(w[t+1]) mu[t+1];  dx = mu;  dw[t] = mu[t] conj(h[t-1]);  dinit = conj(w[0]) mu[0]
+ * `flags` selects outputs with TLPC_NEED_DX / TLPC_NEED_DA (dw) / TLPC_NEED_DZI (dinit);
+ * query the workspace with the same flags.
+ */
+int tlpc_scan_backward(int dtype, const void* g, const void* w, const void* init, const void* h,
+                       void* dx, void* dw, void* dinit, int64_t B, int64_t T, int32_t flags,
+                       int32_t chunk_len, void* workspace, size_t workspace_bytes,
+                       void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCHLPC_AMD_H */

@@ -1,0 +1,211 @@
+"""HIP kernels vs the FP64 oracle: forward, fused adjoint, first-order scan.
+
+Inputs are seeded; the oracle runs in FP64 on the same (cast) inputs. Error is
+the relative L2 norm ||out - ref|| / ||ref|| per output tensor; the bars are
+FP32/complex64 <= 1e-5 and FP64/complex128 <= 1e-12.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.gpu_util import REL_TOL, dev, host, rel, stable_inputs, up
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.float32, np.float64, np.complex64, np.complex128]
+
+
+def ops():
+    import torchlpc_amd  # noqa: F401
+
+    return torch.ops.torchlpc_amd
+
+
+def fwd_case(seed, B, T, N, dtype, strategy=0, chunk=0):
+    rng = np.random.default_rng(seed)
+    x, A, zi = stable_inputs(rng, B, T, N, dtype)
+    y = host(ops().lpc(dev(x), dev(A), dev(zi), strategy, chunk))
+    ref = O.lpc(up(x, dtype), up(A, dtype), up(zi, dtype))
+    return y, ref
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 8, 12, 16, 17, 31, 32, 33])
+def test_forward_orders(dtype, N):
+    for T in (1, 7, 100, 1021):
+        y, ref = fwd_case(N * 100 + T, 3, T, N, dtype)
+        assert rel(y, ref) <= REL_TOL[dtype], (T, rel(y, ref))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("N,chunk", [(2, 16), (2, 48), (4, 32), (16, 16), (16, 64), (16, 272), (32, 32), (32, 96)])
+def test_forward_chunk_lengths(dtype, N, chunk):
+    for T in (chunk - 1, chunk, chunk + 1, 5 * chunk + 3, 4097):
+        if T < 1:
+            continue
+        y, ref = fwd_case(T + chunk, 2, T, N, dtype, strategy=1, chunk=chunk)
+        assert rel(y, ref) <= REL_TOL[dtype], (T, rel(y, ref))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_forward_serial_strategy(dtype):
+    for N in (2, 16, 40):
+        y, ref = fwd_case(N, 3, 333, N, dtype, strategy=3)
+        assert rel(y, ref) <= REL_TOL[dtype]
+
+
+def bwd_case(seed, B, T, N, dtype, need=(True, True, True), strategy=0, chunk=0):
+    rng = np.random.default_rng(seed)
+    x, A, zi = stable_inputs(rng, B, T, N, dtype)
+    g = stable_inputs(rng, B, T, 1, dtype)[0]
+    yd = ops().lpc(dev(x), dev(A), dev(zi), strategy, chunk)
+    dx, dA, dzi = ops().lpc_backward(dev(g), dev(A), dev(zi), yd, *need, strategy, chunk)
+    y64 = O.lpc(up(x, dtype), up(A, dtype), up(zi, dtype))
+    ref = O.lpc_backward(up(g, dtype), up(A, dtype), up(zi, dtype), y64)
+    return (host(dx), host(dA), host(dzi)), ref
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 8, 16, 17, 32, 33])
+def test_backward_orders(dtype, N):
+    for T in (1, 9, 100, 1021):
+        (dx, dA, dzi), (rx, rA, rzi) = bwd_case(N * 7 + T, 3, T, N, dtype)
+        tol = REL_TOL[dtype]
+        assert rel(dx, rx) <= tol, ("dx", T, rel(dx, rx))
+        assert rel(dA, rA) <= tol, ("dA", T, rel(dA, rA))
+        assert rel(dzi, rzi) <= tol, ("dzi", T, rel(dzi, rzi))
+
+
+@pytest.mark.parametrize("N,chunk", [(2, 16), (4, 48), (16, 16), (16, 112), (32, 64)])
+def test_backward_chunk_lengths(N, chunk):
+    for T in (chunk - 1, chunk + 1, 7 * chunk + 5):
+        if T < 1:
+            continue
+        (dx, dA, dzi), (rx, rA, rzi) = bwd_case(T, 2, T, N, np.float64, strategy=1, chunk=chunk)
+        assert rel(dx, rx) <= 1e-12 and rel(dA, rA) <= 1e-12 and rel(dzi, rzi) <= 1e-12
+
+
+@pytest.mark.parametrize("need", [(True, False, False), (False, True, False), (False, False, True), (True, True, False)])
+def test_backward_need_flags(need):
+    (dx, dA, dzi), (rx, rA, rzi) = bwd_case(5, 2, 300, 4, np.float32, need=need)
+    for flag, out, ref in zip(need, (dx, dA, dzi), (rx, rA, rzi)):
+        if flag:
+            assert rel(out, ref) <= 1e-5
+        else:
+            assert out.size == 0
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_backward_serial_strategy(dtype):
+    for N in (3, 40):
+        (dx, dA, dzi), (rx, rA, rzi) = bwd_case(N, 2, 200, N, dtype, strategy=3)
+        tol = REL_TOL[dtype]
+        assert rel(dx, rx) <= tol and rel(dA, rA) <= tol and rel(dzi, rzi) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("N", [2, 16, 32])
+def test_time_invariant(dtype, N):
+    rng = np.random.default_rng(N)
+    B, T = 4, 3000
+    a = O.step_up(rng.uniform(-0.8, 0.8, (B, N)))
+    x = rng.standard_normal((B, T))
+    zi = rng.standard_normal((B, N))
+    g = rng.standard_normal((B, T))
+    a_, x_, zi_, g_ = (v.astype(dtype) for v in (a, x, zi, g))
+    A3 = np.ascontiguousarray(np.broadcast_to(a_[:, None, :], (B, T, N)))
+    y64 = O.lpc(up(x_, dtype), up(A3, dtype), up(zi_, dtype))
+    rx, rA, rzi = O.lpc_backward(up(g_, dtype), up(A3, dtype), up(zi_, dtype), y64)
+    # 2-D a[B,N]: reduced gradient
+    y = ops().lpc(dev(x_), dev(a_), dev(zi_))
+    assert rel(host(y), y64) <= REL_TOL[dtype]
+    dx, da, dzi = ops().lpc_backward(dev(g_), dev(a_), dev(zi_), y, True, True, True)
+    assert rel(host(dx), rx) <= REL_TOL[dtype]
+    assert rel(host(da), rA.sum(1)) <= REL_TOL[dtype]
+    assert rel(host(dzi), rzi) <= REL_TOL[dtype]
+    # time-broadcast 3-D view: read without materialising, per-sample gradient
+    Aexp = dev(a_)[:, None, :].expand(B, T, N)
+    y2 = ops().lpc(dev(x_), Aexp, dev(zi_))
+    assert torch.equal(y2, y)
+    _, dA, _ = ops().lpc_backward(dev(g_), Aexp, dev(zi_), y2, False, True, False)
+    assert dA.shape == (B, T, N)
+    assert rel(host(dA), rA) <= REL_TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_scan(dtype):
+    rng = np.random.default_rng(3)
+    B, T = 3, 5000
+    x = rng.standard_normal((B, T))
+    w = rng.uniform(-0.99, 0.99, (B, T))
+    init = rng.standard_normal(B)
+    g = rng.standard_normal((B, T))
+    if np.issubdtype(dtype, np.complexfloating):
+        w = w * np.exp(1j * rng.uniform(0, 2 * np.pi, (B, T)))
+        x = x + 1j * rng.standard_normal((B, T))
+    x, w, init, g = (v.astype(dtype) for v in (x, w, init, g))
+    h = ops().scan(dev(x), dev(w), dev(init))
+    h64 = O.scan(up(x, dtype), up(w, dtype), up(init, dtype))
+    assert rel(host(h), h64) <= REL_TOL[dtype]
+    dx, dw, dinit = ops().scan_backward(dev(g), dev(w), dev(init), h, True, True, True)
+    rx, rw, rinit = O.scan_backward(up(g, dtype), up(w, dtype), up(init, dtype), h64)
+    assert rel(host(dx), rx) <= REL_TOL[dtype]
+    assert rel(host(dw), rw) <= REL_TOL[dtype]
+    assert rel(host(dinit), rinit) <= REL_TOL[dtype]
+
+
+def test_scan_geometric_known_answer():
+    # decay 0.9, impulse at t=0 -> 0.9**t (the reference's parallel-scan demo)
+    B, T = 16, 20480
+    w = torch.full((B, T), 0.9, device="cuda")
+    x = torch.zeros(B, T, device="cuda")
+    x[:, 0] = 1.0
+    h = ops().scan(x, w, torch.zeros(B, device="cuda"))
+    # exact powers of the fp32 value of 0.9; repeated fp32 products drift by ~t ulp
+    ref = np.float64(np.float32(0.9)) ** np.arange(T, dtype=np.float64)
+    t = np.arange(T)
+    err = np.abs(host(h) - ref)
+    assert np.all(err <= (t + 1) * 1.2e-7 * ref + 1e-37)
+    np.testing.assert_allclose(host(h)[:, :64], np.broadcast_to(ref[:64], (B, 64)), rtol=1e-5)
+
+
+def test_deterministic():
+    rng = np.random.default_rng(1)
+    x, A, zi = stable_inputs(rng, 8, 20000, 16, np.float32)
+    g = rng.standard_normal((8, 20000)).astype(np.float32)
+    X, AA, Z, G = dev(x), dev(A), dev(zi), dev(g)
+    y1 = ops().lpc(X, AA, Z)
+    y2 = ops().lpc(X, AA, Z)
+    assert torch.equal(y1, y2)
+    b1 = ops().lpc_backward(G, AA, Z, y1, True, True, True)
+    b2 = ops().lpc_backward(G, AA, Z, y1, True, True, True)
+    for u, v in zip(b1, b2):
+        assert torch.equal(u, v)
+
+
+def test_zi_zf_chaining():
+    import torchlpc_amd
+
+    rng = np.random.default_rng(2)
+    x, A, zi = stable_inputs(rng, 4, 5000, 6, np.float64)
+    X, AA, Z = dev(x), dev(A), dev(zi)
+    yfull = torchlpc_amd.sample_wise_lpc(X, AA, Z)
+    y1, zf = torchlpc_amd.sample_wise_lpc(X[:, :1234], AA[:, :1234], Z, return_zf=True)
+    y2 = torchlpc_amd.sample_wise_lpc(X[:, 1234:], AA[:, 1234:], zf)
+    assert rel(host(torch.cat([y1, y2], 1)), host(yfull)) <= 1e-13
+
+
+def test_reference_cpu_op_fp32_agreement():
+    """GPU fp32 vs the reference's own compiled CPU op (fp32) on config-1-shaped inputs."""
+    if not O.REF_LIB_PATH.exists():
+        pytest.skip("reference CPU op not built")
+    torch.ops.load_library(str(O.REF_LIB_PATH))
+    x, A, zi = (v.astype(np.float32) for v in O.make_config1())
+    ref32 = torch.ops.torchlpc.lpc(torch.from_numpy(x), torch.from_numpy(A), torch.from_numpy(zi)).numpy()
+    y = host(ops().lpc(dev(x), dev(A), dev(zi)))
+    y64 = O.lpc(x.astype(np.float64), A.astype(np.float64), zi.astype(np.float64))
+    assert rel(y, y64) <= 1e-5
+    assert rel(y, ref32) <= 1e-5

@@ -1,0 +1,116 @@
+"""Host-side logic of the drop-in API (CPU only, no kernel runs).
+
+Shapes and routing are checked on the meta device (the ops have fake kernels);
+CPU tensors must raise a clear error rather than fall back to a CPU path.
+"""
+from __future__ import annotations
+
+import pytest
+import torch
+
+import torchlpc_amd
+from torchlpc_amd import LPC, linear_recurrence, sample_wise_lpc
+
+
+def meta(*shape, dtype=torch.float32):
+    return torch.empty(*shape, device="meta", dtype=dtype)
+
+
+def test_extension_loaded_and_single_hip_runtime():
+    assert torchlpc_amd.EXTENSION_LOADED, torchlpc_amd.LOAD_ERROR
+    libs = torchlpc_amd.hip_runtimes_mapped()
+    assert len(libs) == 1, libs
+
+
+@pytest.mark.parametrize("B,T,N", [(2, 50, 3), (4, 7, 16), (1, 3, 5)])
+def test_shapes_and_zf_convention(B, T, N):
+    x, A, zi = meta(B, T), meta(B, T, N), meta(B, N)
+    y = sample_wise_lpc(x, A, zi)
+    assert y.shape == (B, T) and y.device.type == "meta"
+    y, zf = sample_wise_lpc(x, A, zi, return_zf=True)
+    # zf = y[:, -N:].flip(1): (B, N), or (B, T) when T < N (reference quirk)
+    assert zf.shape == (B, min(N, T))
+
+
+def test_time_invariant_shapes():
+    y = sample_wise_lpc(meta(3, 40), meta(3, 4))
+    assert y.shape == (3, 40)
+    y = sample_wise_lpc(meta(3, 40), meta(3, 1, 4).expand(3, 40, 4))
+    assert y.shape == (3, 40)
+
+
+def test_order_one_routes_to_recurrence():
+    y = sample_wise_lpc(meta(2, 30), meta(2, 30, 1), meta(2, 1))
+    assert y.shape == (2, 30)
+    assert y.grad_fn is None  # meta inputs without grad
+    x = meta(2, 30).requires_grad_()
+    y = sample_wise_lpc(x, meta(2, 30, 1))
+    assert type(y.grad_fn).__name__ == "RecurrenceBackward"
+    y = sample_wise_lpc(x, meta(2, 30, 3))
+    assert type(y.grad_fn).__name__ == "LPCBackward"
+
+
+def test_shape_asserts_match_reference():
+    with pytest.raises(AssertionError):
+        sample_wise_lpc(meta(2, 30), meta(3, 30, 2))
+    with pytest.raises(AssertionError):
+        sample_wise_lpc(meta(2, 30), meta(2, 31, 2))
+    with pytest.raises(AssertionError):
+        sample_wise_lpc(meta(2, 30), meta(2, 30, 2), meta(2, 3))
+    with pytest.raises(AssertionError):
+        linear_recurrence(meta(2, 30), meta(2, 31))
+
+
+def test_cpu_tensors_raise_no_fallback():
+    x, A, zi = torch.randn(2, 16), torch.randn(2, 16, 2) * 0.1, torch.zeros(2, 2)
+    with pytest.raises(RuntimeError, match="CPU tensors are not supported"):
+        sample_wise_lpc(x, A, zi)
+    with pytest.raises(RuntimeError, match="CPU tensors are not supported"):
+        linear_recurrence(torch.rand(2, 16), x)
+
+
+def test_meta_backward_shapes():
+    x = meta(3, 20).requires_grad_()
+    A = meta(3, 20, 4).requires_grad_()
+    zi = meta(3, 4).requires_grad_()
+    y = LPC.apply(x, A, zi)
+    gx, gA, gzi = torch.autograd.grad(y.sum(), (x, A, zi))
+    assert gx.shape == x.shape and gA.shape == A.shape and gzi.shape == zi.shape
+    a2 = meta(3, 4).requires_grad_()
+    y = LPC.apply(x, a2, zi)
+    (ga,) = torch.autograd.grad(y.sum(), (a2,))
+    assert ga.shape == (3, 4)
+
+
+def test_shifted_coefficients_identity():
+    # S[s, i-1] = A[s-N+i, i-1]
+    from torchlpc_amd.core import lagged_outputs, shifted_coefficients
+
+    B, T, N = 2, 7, 3
+    A = torch.arange(B * T * N, dtype=torch.float64).reshape(B, T, N) + 1
+    S = shifted_coefficients(A)
+    for s in range(T + N):
+        for i in range(1, N + 1):
+            t = s - N + i
+            want = A[:, t, i - 1] if 0 <= t < T else torch.zeros(B, dtype=A.dtype)
+            assert torch.equal(S[:, s, i - 1], want)
+    y = torch.randn(B, T, dtype=torch.float64)
+    zi = torch.randn(B, N, dtype=torch.float64)
+    Y = lagged_outputs(y, zi, N)
+    for t in range(T):
+        for i in range(1, N + 1):
+            want = y[:, t - i] if t - i >= 0 else zi[:, i - t - 1]
+            assert torch.equal(Y[:, t, i - 1], want)
+
+
+def test_autotune_table_roundtrip(tmp_path, monkeypatch):
+    from torchlpc_amd import autotune
+
+    monkeypatch.setenv("TORCHLPC_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    autotune.clear()
+    x, A = meta(4, 1000), meta(4, 1000, 16)
+    assert autotune.lookup("lpc_forward", x, A) == (0, 0)
+    autotune.store("lpc_forward", x, A, (1, 256))
+    assert autotune.lookup("lpc_forward", x, A) == (1, 256)
+    assert (tmp_path / "tune.json").exists()
+    autotune.clear()

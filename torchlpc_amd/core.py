@@ -1,0 +1,128 @@
+"""``LPC`` autograd Function: the drop-in for ``torchlpc.core.LPC``.
+
+Reference: ``torchlpc/core.py:159-265``. Same ``apply(x, A, zi)`` signature,
+``setup_context`` saving ``(A, zi, y)``, ``backward``, forward-mode ``jvp`` and a
+custom ``vmap``. The arithmetic runs in the HIP kernels of
+``libtorchlpc_amd_hip.so``:
+
+* forward: ``torch.ops.torchlpc_amd.lpc`` (chunked scan or per-signal kernel);
+* backward: ``torch.ops.torchlpc_amd.lpc_backward``, the fused adjoint that gives
+  dx, dA and dzi in one reverse-time pass. When the backward itself is being
+  differentiated (``create_graph=True``), the gradients are instead expressed
+  through ``LPC.apply`` (one more forward filter on the reversed, shifted
+  coefficients, as the reference does at ``core.py:194-227``) so that they stay
+  differentiable.
+
+Besides ``A[B,T,N]`` the Function accepts a time-invariant ``a[B,N]`` (its
+gradient is then ``[B,N]``); a time-broadcast ``A`` (``A.stride(1) == 0``) is
+read without being materialised.
+"""
+from __future__ import annotations
+
+from typing import Any, List, Optional, Tuple
+
+import torch
+import torch.nn.functional as F
+from torch.autograd import Function
+
+from . import autotune
+from ._ext import ops
+
+
+def lagged_outputs(y: torch.Tensor, zi: torch.Tensor, order: int) -> torch.Tensor:
+    """Y[b, t, i-1] = y[b, t-i] for i = 1..order, with y[b, -k] = zi[b, k-1]."""
+    T = y.shape[1]
+    ext = torch.cat([zi.flip(1), y], dim=1)  # ext[:, order + u] = y_u
+    t = torch.arange(T, device=y.device)[:, None]
+    i = torch.arange(1, order + 1, device=y.device)[None, :]
+    return ext[:, t - i + order]
+
+
+def shifted_coefficients(A: torch.Tensor) -> torch.Tensor:
+    """S[b, s, i-1] = A[b, s-N+i, i-1] for s in [0, T+N) (zero outside [0, T)).
+
+    Row s of S holds the coefficients that multiply lambda[s-N+i] in the adjoint
+    recurrence written in direct form (reference: the flip/pad/reshape at
+    ``torchlpc/core.py:194-201``).
+    """
+    N = A.shape[2]
+    cols = [F.pad(A[:, :, i - 1], (N - i, i)) for i in range(1, N + 1)]
+    return torch.stack(cols, dim=2)
+
+
+def _as_3d(A: torch.Tensor, T: int) -> torch.Tensor:
+    return A[:, None, :].expand(A.shape[0], T, A.shape[1]) if A.dim() == 2 else A
+
+
+def composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi):
+    """Differentiable adjoint built from ``LPC.apply`` (used under create_graph)."""
+    B, T = grad_y.shape
+    A3 = _as_3d(A, T)
+    N = A3.shape[2]
+    S = shifted_coefficients(A3)  # [B, T+N, N]
+    g_ext = F.pad(grad_y, (N, 0))  # zero gradient at positions -N..-1
+    lam_rev = LPC.apply(g_ext.flip(1), S.flip(1).conj(), torch.zeros_like(zi))
+    dzi = lam_rev[:, T:] if need_zi else None
+    lam = lam_rev[:, :T].flip(1)
+    dx = lam if need_x else None
+    dA = None
+    if need_A:
+        dA = -(lagged_outputs(y, zi, N).conj() * lam.unsqueeze(2))
+        if A.dim() == 2:
+            dA = dA.sum(1)
+    return dx, dA, dzi
+
+
+class LPC(Function):
+    """y[t] = x[t] - sum_{i=1..N} A[t,i] y[t-i], with y[-k] = zi[k-1]."""
+
+    @staticmethod
+    def forward(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> torch.Tensor:
+        strategy, chunk = autotune.lookup("lpc_forward", x, A)
+        return ops().lpc(x, A, zi, strategy, chunk)
+
+    @staticmethod
+    def setup_context(ctx: Any, inputs: List[Any], output: Any) -> Any:
+        _, A, zi = inputs
+        ctx.save_for_backward(A, zi, output)
+        ctx.save_for_forward(A, zi, output)
+
+    @staticmethod
+    def backward(
+        ctx: Any, grad_y: torch.Tensor
+    ) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor], Optional[torch.Tensor]]:
+        A, zi, y = ctx.saved_tensors
+        need_x, need_A, need_zi = ctx.needs_input_grad[:3]
+        if torch.is_grad_enabled():
+            return composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi)
+        strategy, chunk = autotune.lookup("lpc_backward", grad_y, A)
+        dx, dA, dzi = ops().lpc_backward(
+            grad_y, A, zi, y, need_x, need_A, need_zi, strategy, chunk
+        )
+        return (dx if need_x else None, dA if need_A else None, dzi if need_zi else None)
+
+    @staticmethod
+    def jvp(
+        ctx: Any,
+        grad_x: Optional[torch.Tensor],
+        grad_A: Optional[torch.Tensor],
+        grad_zi: Optional[torch.Tensor],
+    ) -> torch.Tensor:
+        A, zi, y = ctx.saved_tensors
+        N = A.shape[-1]
+        tx = grad_x if grad_x is not None else torch.zeros_like(y)
+        tzi = grad_zi if grad_zi is not None else torch.zeros_like(zi)
+        if grad_A is not None:
+            gA = _as_3d(grad_A, y.shape[1])
+            tx = tx - (lagged_outputs(y, zi, N) * gA).sum(2)
+        return LPC.apply(tx, A, tzi)
+
+    @staticmethod
+    def vmap(info, in_dims, x, A, zi):
+        def front(t, d):
+            return t.expand(info.batch_size, *t.shape) if d is None else t.movedim(d, 0)
+
+        args = [front(t, d) for t, d in zip((x, A, zi), in_dims)]
+        flat = [t.reshape(-1, *t.shape[2:]) for t in args]
+        y = LPC.apply(*flat)
+        return y.reshape(info.batch_size, -1, *y.shape[1:]), 0

@@ -1,0 +1,269 @@
+// C ABI of libtorchlpc_amd_hip.so (declared in include/torchlpc_amd.h).
+// Validates sizes, carves the caller's workspace, picks a strategy and launches.
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/torchlpc_amd.h"
+#include "tlpc_kernels.h"
+
+namespace tlpc {
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+inline int compiled_order(int n) {
+  if (n <= 1) return 1;
+  if (n <= 2) return 2;
+  if (n <= 4) return 4;
+  if (n <= 8) return 8;
+  if (n <= 16) return 16;
+  if (n <= 32) return 32;
+  return 0;  // served by the serial strategy
+}
+
+inline size_t elem_size(int dtype) {
+  switch (dtype) {
+    case TLPC_F32: return 4;
+    case TLPC_F64: return 8;
+    case TLPC_C64: return 8;
+    case TLPC_C128: return 16;
+    default: return 0;
+  }
+}
+
+// Chunk length: a multiple of the staging block, sized so that the rerun phase
+// has about 32K lanes in flight, capped at 4096 samples.
+inline int pick_chunk(int64_t B, int64_t T, int np, int requested) {
+  const int blk = chunk_block(np);
+  int64_t L = requested > 0 ? requested : (B * T + 32767) / 32768;
+  if (L < blk) L = blk;
+  if (L > 4096 && requested <= 0) L = 4096;
+  if (L > (int64_t)1 << 24) L = (int64_t)1 << 24;
+  L = (L + blk - 1) / blk * blk;
+  return (int)L;
+}
+
+inline int pick_strategy(int strategy, int np) {
+  if (np == 0) return TLPC_STRATEGY_SERIAL;
+  if (strategy == TLPC_STRATEGY_SERIAL) return TLPC_STRATEGY_SERIAL;
+  return TLPC_STRATEGY_CHUNKED;
+}
+
+struct Plan {
+  int np = 0;
+  int strategy = 0;
+  int L = 0;
+  int64_t P = 0;
+  size_t summ_off = 0, carry_off = 0, dapart_off = 0, total = 0;
+};
+
+Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
+               int32_t chunk_len, bool backward) {
+  Plan p;
+  const size_t es = elem_size(dtype);
+  p.np = compiled_order(n);
+  p.strategy = pick_strategy(strategy, p.np);
+  if (p.strategy == TLPC_STRATEGY_SERIAL) {
+    // the serial adjoint keeps lambda in the workspace when dx is not requested
+    if (backward && !(flags & TLPC_NEED_DX)) p.total = align_up((size_t)B * T * es);
+    p.carry_off = 0;
+    return p;
+  }
+  p.L = pick_chunk(B, T, p.np, chunk_len);
+  p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
+  const size_t summ = (size_t)B * (p.P > 1 ? p.P - 1 : 0) * (p.np * p.np + p.np) * es;
+  const size_t carry = (size_t)B * p.P * p.np * es;
+  const size_t dapart =
+      (backward && (flags & TLPC_TIME_INVARIANT) && (flags & TLPC_NEED_DA) && !(flags & TLPC_DA_FULL))
+          ? (size_t)B * p.P * p.np * es
+          : 0;
+  p.summ_off = 0;
+  p.carry_off = align_up(summ);
+  p.dapart_off = p.carry_off + align_up(carry);
+  p.total = p.dapart_off + align_up(dapart);
+  return p;
+}
+
+int check_common(int dtype, int64_t B, int64_t T, int32_t n) {
+  if (elem_size(dtype) == 0) return TLPC_EDTYPE;
+  if (B < 0 || T < 0 || n < 1) return TLPC_EINVAL;
+  if (n > 4096) return TLPC_EORDER;
+  return TLPC_OK;
+}
+
+template <typename S, int NP>
+int run_chunked(const ChunkArgs<S>& a, bool backward, hipStream_t st) {
+  return backward ? launch_chunked_bwd<S, NP>(a, st) : launch_chunked_fwd<S, NP>(a, st);
+}
+
+template <typename S>
+int dispatch_np(const Plan& p, const ChunkArgs<S>& a, bool backward, hipStream_t st) {
+  switch (p.np) {
+    case 1: return run_chunked<S, 1>(a, backward, st);
+    case 2: return run_chunked<S, 2>(a, backward, st);
+    case 4: return run_chunked<S, 4>(a, backward, st);
+    case 8: return run_chunked<S, 8>(a, backward, st);
+    case 16: return run_chunked<S, 16>(a, backward, st);
+    case 32: return run_chunked<S, 32>(a, backward, st);
+    default: return TLPC_EORDER;
+  }
+}
+
+template <typename S>
+int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* dx, void* dA,
+        void* dzi, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
+        int32_t chunk_len, void* ws, size_t ws_bytes, void* stream, bool backward, int neg) {
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const Plan p = make_plan(dtype, B, T, n, flags, strategy, chunk_len, backward);
+  if (B == 0) return TLPC_OK;
+  const bool ti = (flags & TLPC_TIME_INVARIANT) != 0;
+  const bool da_reduced = ti && !(flags & TLPC_DA_FULL);
+  const bool need_dx = backward && (flags & TLPC_NEED_DX);
+  const bool need_da = backward && (flags & TLPC_NEED_DA);
+  const bool need_dzi = backward && (flags & TLPC_NEED_DZI);
+  if (!A || (!x && T > 0)) return TLPC_EINVAL;
+  if (!backward && !y && T > 0) return TLPC_EINVAL;
+  if (backward && need_da && !y && T > 0) return TLPC_EINVAL;
+  if ((need_dx && !dx) || (need_da && !dA) || (need_dzi && !dzi)) return TLPC_EINVAL;
+  if (ws_bytes < p.total || (p.total > 0 && !ws)) return TLPC_EWORKSPACE;
+
+  if (T == 0) {
+    // nothing flows backwards through an empty signal: dzi and da are zero
+    int rc = TLPC_OK;
+    if (need_dzi) rc = launch_fill_zero<S>(static_cast<S*>(dzi), B * n, st);
+    if (rc == TLPC_OK && need_da && da_reduced) rc = launch_fill_zero<S>(static_cast<S*>(dA), B * n, st);
+    return rc;
+  }
+
+  ChunkArgs<S> a{};
+  a.x = static_cast<const S*>(x);
+  a.A = static_cast<const S*>(A);
+  a.zi = static_cast<const S*>(zi);
+  a.y = static_cast<S*>(y);
+  a.dx = need_dx ? static_cast<S*>(dx) : nullptr;
+  a.dzi = need_dzi ? static_cast<S*>(dzi) : nullptr;
+  a.B = B;
+  a.T = T;
+  a.n = n;
+  a.L = p.L;
+  a.P = p.P;
+  a.a_bstride = ti ? n : T * n;
+  a.a_tstride = ti ? 0 : n;
+  a.neg = neg;
+  char* w = static_cast<char*>(ws);
+  if (p.strategy == TLPC_STRATEGY_SERIAL) {
+    a.dA = (need_da && !da_reduced) ? static_cast<S*>(dA) : nullptr;
+    a.da_ti = (need_da && da_reduced) ? static_cast<S*>(dA) : nullptr;
+    a.carry = reinterpret_cast<S*>(w);
+    return backward ? launch_serial_bwd<S>(a, st) : launch_serial_fwd<S>(a, st);
+  }
+  a.summ = reinterpret_cast<S*>(w + p.summ_off);
+  a.carry = reinterpret_cast<S*>(w + p.carry_off);
+  if (need_da && da_reduced) {
+    a.dapart = reinterpret_cast<S*>(w + p.dapart_off);
+    a.da_ti = static_cast<S*>(dA);
+  } else if (need_da) {
+    a.dA = static_cast<S*>(dA);
+  }
+  return dispatch_np<S>(p, a, backward, st);
+}
+
+int run_dtype(int dtype, const void* x, const void* A, const void* zi, void* y, void* dx,
+              void* dA, void* dzi, int64_t B, int64_t T, int32_t n, int32_t flags,
+              int32_t strategy, int32_t chunk_len, void* ws, size_t ws_bytes, void* stream,
+              bool backward, int neg = 0) {
+  const int rc = check_common(dtype, B, T, n);
+  if (rc != TLPC_OK) return rc;
+  switch (dtype) {
+    case TLPC_F32:
+      return run<float>(dtype, x, A, zi, y, dx, dA, dzi, B, T, n, flags, strategy, chunk_len, ws,
+                        ws_bytes, stream, backward, neg);
+    case TLPC_F64:
+      return run<double>(dtype, x, A, zi, y, dx, dA, dzi, B, T, n, flags, strategy, chunk_len, ws,
+                         ws_bytes, stream, backward, neg);
+    case TLPC_C64:
+      return run<Cx<float>>(dtype, x, A, zi, y, dx, dA, dzi, B, T, n, flags, strategy, chunk_len,
+                            ws, ws_bytes, stream, backward, neg);
+    case TLPC_C128:
+      return run<Cx<double>>(dtype, x, A, zi, y, dx, dA, dzi, B, T, n, flags, strategy, chunk_len,
+                             ws, ws_bytes, stream, backward, neg);
+    default:
+      return TLPC_EDTYPE;
+  }
+}
+
+}  // namespace
+}  // namespace tlpc
+
+using namespace tlpc;
+
+#define TLPC_EXPORT extern "C" __attribute__((visibility("default")))
+
+TLPC_EXPORT const char* tlpc_error_string(int code) {
+  switch (code) {
+    case TLPC_OK: return "ok";
+    case TLPC_EINVAL: return "invalid argument";
+    case TLPC_EDTYPE: return "unsupported dtype";
+    case TLPC_EWORKSPACE: return "workspace too small";
+    case TLPC_ELAUNCH: return "kernel launch failed";
+    case TLPC_EORDER: return "unsupported filter order";
+    default: return "unknown error";
+  }
+}
+
+TLPC_EXPORT int tlpc_abi_version(void) { return 1; }
+
+TLPC_EXPORT size_t tlpc_lpc_forward_workspace(int dtype, int64_t B, int64_t T, int32_t N,
+                                              int32_t flags, int32_t strategy, int32_t chunk_len) {
+  if (check_common(dtype, B, T, N) != TLPC_OK) return 0;
+  return make_plan(dtype, B, T, N, flags, strategy, chunk_len, false).total;
+}
+
+TLPC_EXPORT int tlpc_lpc_forward(int dtype, const void* x, const void* A, const void* zi, void* y,
+                                 int64_t B, int64_t T, int32_t N, int32_t flags, int32_t strategy,
+                                 int32_t chunk_len, void* workspace, size_t workspace_bytes,
+                                 void* hip_stream) {
+  return run_dtype(dtype, x, A, zi, y, nullptr, nullptr, nullptr, B, T, N,
+                   flags & TLPC_TIME_INVARIANT, strategy, chunk_len, workspace, workspace_bytes,
+                   hip_stream, false);
+}
+
+TLPC_EXPORT size_t tlpc_lpc_backward_workspace(int dtype, int64_t B, int64_t T, int32_t N,
+                                               int32_t flags, int32_t strategy, int32_t chunk_len) {
+  if (check_common(dtype, B, T, N) != TLPC_OK) return 0;
+  return make_plan(dtype, B, T, N, flags, strategy, chunk_len, true).total;
+}
+
+TLPC_EXPORT int tlpc_lpc_backward(int dtype, const void* g, const void* A, const void* zi,
+                                  const void* y, void* dx, void* dA, void* dzi, int64_t B,
+                                  int64_t T, int32_t N, int32_t flags, int32_t strategy,
+                                  int32_t chunk_len, void* workspace, size_t workspace_bytes,
+                                  void* hip_stream) {
+  return run_dtype(dtype, g, A, zi, const_cast<void*>(y), dx, dA, dzi, B, T, N, flags, strategy,
+                   chunk_len, workspace, workspace_bytes, hip_stream, true);
+}
+
+TLPC_EXPORT size_t tlpc_scan_workspace(int dtype, int64_t B, int64_t T, int32_t flags,
+                                       int32_t chunk_len) {
+  if (check_common(dtype, B, T, 1) != TLPC_OK) return 0;
+  return make_plan(dtype, B, T, 1, flags, TLPC_STRATEGY_AUTO, chunk_len,
+                   (flags & (TLPC_NEED_DX | TLPC_NEED_DA | TLPC_NEED_DZI)) != 0)
+      .total;
+}
+
+TLPC_EXPORT int tlpc_scan(int dtype, const void* x, const void* w, const void* init, void* h,
+                          int64_t B, int64_t T, int32_t chunk_len, void* workspace,
+                          size_t workspace_bytes, void* hip_stream) {
+  return run_dtype(dtype, x, w, init, h, nullptr, nullptr, nullptr, B, T, 1, 0,
+                   TLPC_STRATEGY_AUTO, chunk_len, workspace, workspace_bytes, hip_stream, false, 1);
+}
+
+TLPC_EXPORT int tlpc_scan_backward(int dtype, const void* g, const void* w, const void* init,
+                                   const void* h, void* dx, void* dw, void* dinit, int64_t B,
+                                   int64_t T, int32_t flags, int32_t chunk_len, void* workspace,
+                                   size_t workspace_bytes, void* hip_stream) {
+  return run_dtype(dtype, g, w, init, const_cast<void*>(h), dx, dw, dinit, B, T, 1,
+                   flags & ~TLPC_TIME_INVARIANT, TLPC_STRATEGY_AUTO, chunk_len, workspace,
+                   workspace_bytes, hip_stream, true, 1);
+}

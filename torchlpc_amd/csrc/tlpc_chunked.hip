@@ -1,0 +1,561 @@
+// Chunked linear-recurrence scan for the all-pole filter (any batch size).
+//
+// Forward recurrence (reference: torchlpc/csrc/scan_cpu.cpp:112-120):
+//   y[t] = x[t] - sum_{i=1..N} A[t,i] y[t-i]
+// The state s_t = [y_{t-1}, ..., y_{t-N}] evolves by a companion matrix, so a chunk
+// of L samples is the affine map s_out = Phi s_in + z. Three phases:
+//   1. summary: per chunk, Phi (N homogeneous basis responses) and z (zero-state
+//      response), computed by N+1 lanes that share the chunk's A rows from LDS;
+//   2. carry:   per signal, s_{c+1} = Phi_c s_c + z_c from s_0 = zi;
+//   3. rerun:   per chunk, one lane re-filters its L samples from the carry and
+//               writes y.
+// The adjoint (reference backward: torchlpc/core.py:186-229) runs the same three
+// phases in reverse time on the transposed recurrence. Its state is the vector of
+// pending sums p_j for positions t-1-j; one step is
+//   lambda[t] = g[t] + p(t);  p(t-j) -= conj(A[t,j]) * lambda[t]   (j = 1..N)
+// and the rerun phase also emits dx = lambda, dA[t,i] = -lambda[t] conj(y[t-i])
+// (reference: core.py:219-227) and dzi (core.py:215-216).
+//
+// NP is the compiled order (a power of two >= the runtime order n); coefficients
+// for lags n < i <= NP are zero, so padded lags never change a result. The state
+// ring is indexed statically: time t lives in slot t mod NP, every chunk starts
+// at a multiple of NP, and the unrolled inner loop covers NP steps.
+#include "tlpc_common.h"
+#include "tlpc_kernels.h"
+
+namespace tlpc {
+
+template <typename S, int NP>
+struct ChunkCfg {
+  static constexpr int TB = NP < 16 ? 16 : NP;  // time steps staged per LDS block
+  static constexpr int LDS_BUDGET = 40 * 1024;
+  static constexpr int G_LANES = 256 / (NP + 1);
+  static constexpr int G_LDS = LDS_BUDGET / (TB * (NP + 1) * (int)sizeof(S));
+  static constexpr int G = (G_LANES < G_LDS ? G_LANES : G_LDS) < 1 ? 1 : (G_LANES < G_LDS ? G_LANES : G_LDS);
+  static constexpr int THREADS = ((G * (NP + 1) + 63) / 64) * 64;
+  static constexpr int ASTRIDE = TB * NP + (16 / (int)sizeof(S) > 0 ? 16 / (int)sizeof(S) : 1);
+  static constexpr int SUMM = NP * NP + NP;  // elements per chunk summary
+};
+
+// ----------------------------------------------------------------------------
+// Staging helper: copy the A rows and the per-step input of G chunks for the
+// time block [t0, t0 + TB) into LDS; rows beyond the signal end and lags >= n
+// are zero.
+template <typename S, int NP>
+__device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, S* sA, S* sX, const S* inp,
+                                            int64_t first, int64_t nchunks, int64_t cpsig,
+                                            int64_t c_base, int t0) {
+  using C = ChunkCfg<S, NP>;
+  const int nthr = blockDim.x;
+  for (int idx = threadIdx.x; idx < C::G * C::TB * NP; idx += nthr) {
+    const int gg = idx / (C::TB * NP);
+    const int r = idx - gg * (C::TB * NP);
+    const int tb = r / NP;
+    const int i = r - tb * NP;
+    const int64_t cid = first + gg;
+    S v = Num<S>::zero();
+    if (cid < nchunks && i < a.n) {
+      const int64_t b = cid / cpsig;
+      const int64_t c = cid - b * cpsig + c_base;
+      const int64_t t = c * a.L + t0 + tb;
+      if (t < a.T) v = coef(a, a.A[b * a.a_bstride + t * a.a_tstride + i]);
+    }
+    sA[gg * C::ASTRIDE + tb * NP + i] = v;
+  }
+  for (int idx = threadIdx.x; idx < C::G * C::TB; idx += nthr) {
+    const int gg = idx / C::TB;
+    const int tb = idx - gg * C::TB;
+    const int64_t cid = first + gg;
+    S v = Num<S>::zero();
+    if (cid < nchunks) {
+      const int64_t b = cid / cpsig;
+      const int64_t c = cid - b * cpsig + c_base;
+      const int64_t t = c * a.L + t0 + tb;
+      if (t < a.T) v = inp[b * a.T + t];
+    }
+    sX[gg * C::TB + tb] = v;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Phase 1 (forward): summaries of chunks c = 0 .. P-2 of every signal.
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
+  using C = ChunkCfg<S, NP>;
+  __shared__ S sA[C::G * C::ASTRIDE];
+  __shared__ S sX[C::G * C::TB];
+  const int64_t cpsig = a.P - 1;
+  const int64_t nchunks = a.B * cpsig;
+  const int64_t first = (int64_t)blockIdx.x * C::G;
+  const int g = threadIdx.x / (NP + 1);
+  const int k = threadIdx.x - g * (NP + 1);  // k < NP: basis e_k; k == NP: zero state
+  const bool active = g < C::G && first + g < nchunks;
+
+  S ring[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) ring[r] = Num<S>::zero();
+  // s_in = e_k means y_{-1-k} = 1, which lives in slot (-1-k) mod NP.
+#pragma unroll
+  for (int r = 0; r < NP; ++r)
+    if (k == r) ring[NP - 1 - r] = Num<S>::one();
+  const S xmask = (k == NP) ? Num<S>::one() : Num<S>::zero();
+
+  for (int t0 = 0; t0 < a.L; t0 += C::TB) {
+    stage_block<S, NP>(a, sA, sX, a.x, first, nchunks, cpsig, 0, t0);
+    __syncthreads();
+    if (active) {
+      const S* rowbase = sA + g * C::ASTRIDE;
+#pragma unroll
+      for (int j = 0; j < C::TB; ++j) {
+        const S* row = rowbase + j * NP;
+        S acc = sX[g * C::TB + j] * xmask;
+#pragma unroll
+        for (int i = NP; i >= 1; --i) acc = Num<S>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
+        ring[j & (NP - 1)] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    S* out = a.summ + (first + g) * C::SUMM;
+    // final state s[r] = y_{L-1-r}, slot (L-1-r) mod NP = NP-1-r
+    if (k < NP) {
+#pragma unroll
+      for (int r = 0; r < NP; ++r) out[r * NP + k] = ring[NP - 1 - r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < NP; ++r) out[NP * NP + r] = ring[NP - 1 - r];
+    }
+  }
+}
+
+// Phase 1 (adjoint): summaries of chunks c = 1 .. P-1 (reverse time).
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
+  using C = ChunkCfg<S, NP>;
+  __shared__ S sA[C::G * C::ASTRIDE];
+  __shared__ S sX[C::G * C::TB];
+  const int64_t cpsig = a.P - 1;
+  const int64_t nchunks = a.B * cpsig;
+  const int64_t first = (int64_t)blockIdx.x * C::G;
+  const int g = threadIdx.x / (NP + 1);
+  const int k = threadIdx.x - g * (NP + 1);
+  const bool active = g < C::G && first + g < nchunks;
+
+  S ring[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) ring[r] = Num<S>::zero();
+  // p_in = e_k: pending sum for position t1-1-k, slot (t1-1-k) mod NP = NP-1-k.
+#pragma unroll
+  for (int r = 0; r < NP; ++r)
+    if (k == r) ring[NP - 1 - r] = Num<S>::one();
+  const S gmask = (k == NP) ? Num<S>::one() : Num<S>::zero();
+
+  for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
+    stage_block<S, NP>(a, sA, sX, a.x, first, nchunks, cpsig, 1, t0);
+    __syncthreads();
+    if (active) {
+      const S* rowbase = sA + g * C::ASTRIDE;
+#pragma unroll
+      for (int jj = 0; jj < C::TB; ++jj) {
+        const int j = C::TB - 1 - jj;  // local time; slot of position t is j mod NP
+        const S* row = rowbase + j * NP;
+        const int h = j & (NP - 1);
+        const S lam = sX[g * C::TB + j] * gmask + ring[h];
+        ring[h] = Num<S>::zero();
+#pragma unroll
+        for (int i = 1; i <= NP; ++i)
+          ring[(j - i) & (NP - 1)] = Num<S>::fms(ring[(j - i) & (NP - 1)], Num<S>::conj(row[i - 1]), lam);
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    S* out = a.summ + (first + g) * C::SUMM;
+    // final pending for position t0-1-r sits in slot NP-1-r
+    if (k < NP) {
+#pragma unroll
+      for (int r = 0; r < NP; ++r) out[r * NP + k] = ring[NP - 1 - r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < NP; ++r) out[NP * NP + r] = ring[NP - 1 - r];
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Phase 2: carries. One wavefront per signal; lane r owns state component r.
+// Forward: carry[c] = state entering chunk c, carry[0] = zi, carry[c+1] = Phi_c carry[c] + z_c.
+// Adjoint: carry[c] = pending sums entering chunk c from the right,
+//          carry[P-1] = 0, carry[c-1] = Psi_c carry[c] + w_c (summary of chunk c at c-1).
+template <typename S, int NP, bool FWD>
+__global__ __launch_bounds__(64) void carry_serial(ChunkArgs<S> a) {
+  const int64_t b = blockIdx.x;
+  const int r = threadIdx.x;
+  const int64_t P = a.P;
+  const int64_t cpsig = P - 1;
+  S s = Num<S>::zero();
+  if (FWD && r < a.n && a.zi) s = a.zi[b * a.n + r];
+  S* carry = a.carry + b * P * NP;
+  const S* summ = a.summ + b * cpsig * ChunkCfg<S, NP>::SUMM;
+  for (int64_t step = 0; step < P; ++step) {
+    const int64_t c = FWD ? step : P - 1 - step;
+    if (r < NP) carry[c * NP + r] = s;
+    if (step == P - 1) break;
+    // summary index: forward chunk c is at c; adjoint chunk c (>=1) is at c-1
+    const S* sm = summ + (FWD ? c : c - 1) * ChunkCfg<S, NP>::SUMM;
+    S acc = (r < NP) ? sm[NP * NP + r] : Num<S>::zero();
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const S sk = shfl(s, k);
+      if (r < NP) acc = Num<S>::fma(acc, sm[r * NP + k], sk);
+    }
+    s = acc;
+  }
+}
+
+// Blocked carry scan for small NP: one workgroup per signal; each thread composes
+// the affine maps of a contiguous run of chunks, the runs are combined by a
+// workgroup scan, then each thread re-walks its run from its carry.
+template <typename S, int NP>
+struct Affine {
+  S M[NP][NP];
+  S v[NP];
+};
+
+template <typename S, int NP>
+__device__ __forceinline__ void affine_identity(Affine<S, NP>& f) {
+#pragma unroll
+  for (int r = 0; r < NP; ++r) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) f.M[r][k] = (r == k) ? Num<S>::one() : Num<S>::zero();
+    f.v[r] = Num<S>::zero();
+  }
+}
+// out = second o first  (apply first, then second)
+template <typename S, int NP>
+__device__ __forceinline__ Affine<S, NP> affine_compose(const Affine<S, NP>& second,
+                                                        const Affine<S, NP>& first) {
+  Affine<S, NP> o;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) {
+    S vv = second.v[r];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) vv = Num<S>::fma(vv, second.M[r][m], first.v[m]);
+    o.v[r] = vv;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      S acc = Num<S>::zero();
+#pragma unroll
+      for (int m = 0; m < NP; ++m) acc = Num<S>::fma(acc, second.M[r][m], first.M[m][k]);
+      o.M[r][k] = acc;
+    }
+  }
+  return o;
+}
+
+template <typename S, int NP>
+__device__ __forceinline__ void load_affine(Affine<S, NP>& f, const S* sm) {
+#pragma unroll
+  for (int r = 0; r < NP; ++r) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) f.M[r][k] = sm[r * NP + k];
+    f.v[r] = sm[NP * NP + r];
+  }
+}
+
+constexpr int kBlockedThreads = 256;
+
+template <typename S, int NP, bool FWD>
+__global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a) {
+  __shared__ Affine<S, NP> sh[kBlockedThreads];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t P = a.P;
+  const int64_t links = P - 1;  // number of affine maps in the chain
+  const int64_t per = (links + kBlockedThreads - 1) / kBlockedThreads;
+  const int64_t lo = tid * per;
+  const int64_t hi = (lo + per < links) ? lo + per : links;
+  const S* summ = a.summ + b * links * ChunkCfg<S, NP>::SUMM;
+  // link l (0-based, in processing order) maps carry of step l to carry of step l+1;
+  // forward: chunk c = l at summary l; adjoint: chunk c = P-1-l at summary P-2-l.
+  Affine<S, NP> f;
+  affine_identity(f);
+  for (int64_t l = lo; l < hi; ++l) {
+    Affine<S, NP> m;
+    load_affine(m, summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM);
+    f = affine_compose(m, f);
+  }
+  sh[tid] = f;
+  __syncthreads();
+  // inclusive Hillis-Steele scan over threads (composition in thread order)
+  for (int off = 1; off < kBlockedThreads; off <<= 1) {
+    Affine<S, NP> other;
+    const bool has = tid >= off;
+    if (has) other = sh[tid - off];
+    __syncthreads();
+    if (has) {
+      f = affine_compose(f, other);
+      sh[tid] = f;
+    }
+    __syncthreads();
+  }
+  // exclusive prefix applied to the initial carry
+  S s[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) s[r] = (FWD && r < a.n && a.zi) ? a.zi[b * a.n + r] : Num<S>::zero();
+  if (tid > 0) {
+    const Affine<S, NP>& pre = sh[tid - 1];
+    S t[NP];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+      S acc = pre.v[r];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) acc = Num<S>::fma(acc, pre.M[r][k], s[k]);
+      t[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < NP; ++r) s[r] = t[r];
+  }
+  S* carry = a.carry + b * P * NP;
+  for (int64_t l = lo; l < hi; ++l) {
+    const int64_t c = FWD ? l : P - 1 - l;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
+    const S* sm = summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM;
+    S t[NP];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+      S acc = sm[NP * NP + r];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) acc = Num<S>::fma(acc, sm[r * NP + k], s[k]);
+      t[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < NP; ++r) s[r] = t[r];
+  }
+  // the thread owning the last link (thread 0 when there is none) writes the final carry
+  if ((lo < hi && hi == links) || (links == 0 && tid == 0)) {
+    const int64_t c = FWD ? P - 1 : 0;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
+  }
+}
+
+// The blocked scan keeps one affine map per thread in LDS; it is used where that
+// fits in 48 KiB.
+template <typename S, int NP>
+constexpr bool blocked_fits() {
+  return (size_t)NP * (NP + 1) * sizeof(S) * kBlockedThreads <= 48 * 1024;
+}
+
+// ----------------------------------------------------------------------------
+// Row loader: the NP coefficients of one time step, zero beyond the order.
+template <typename S, int NP>
+__device__ __forceinline__ void load_row(S (&row)[NP], const ChunkArgs<S>& a, const S* p) {
+  if (a.n == NP) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) row[i] = coef(a, p[i]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) row[i] = (i < a.n) ? coef(a, p[i]) : Num<S>::zero();
+  }
+}
+
+// Phase 3 (forward): one lane per chunk re-filters from its carry and writes y.
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
+  const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
+  const int64_t t0 = c * a.L;
+  const int64_t rem = a.T - t0;
+  const int len = rem < a.L ? (int)rem : a.L;
+  S ring[NP];
+  const S* cr = a.carry + cid * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
+  const S* xb = a.x + b * a.T;
+  const S* Ab = a.A + b * a.a_bstride;
+  S* yb = a.y + b * a.T;
+  for (int jb = 0; jb < len; jb += NP) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      if (jb + j < len) {
+        const int64_t t = t0 + jb + j;
+        S row[NP];
+        load_row<S, NP>(row, a, Ab + t * a.a_tstride);
+        S acc = xb[t];
+#pragma unroll
+        for (int i = NP; i >= 1; --i) acc = Num<S>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
+        ring[j] = acc;
+        yb[t] = acc;
+      }
+    }
+  }
+}
+
+// Phase 3 (adjoint): one lane per chunk walks its samples backwards from the carry,
+// writes dx and dA (time-varying) or a per-chunk partial of da (time-invariant),
+// and chunk 0 writes dzi.
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
+  const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
+  const int64_t t0 = c * a.L;
+  S ring[NP];  // pending sums, slot u mod NP for position u
+  const S* cr = a.carry + cid * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
+  const S* gb = a.x + b * a.T;  // upstream gradient
+  const S* Ab = a.A + b * a.a_bstride;
+  const S* yb = a.y + b * a.T;
+  const S* zib = a.zi ? a.zi + b * a.n : nullptr;
+  S* dxb = a.dx ? a.dx + b * a.T : nullptr;
+  S* dAb = a.dA ? a.dA + b * a.T * a.n : nullptr;
+  const bool need_da = a.dA != nullptr || a.dapart != nullptr;
+  // y ring: slot u mod NP holds y_u (zi for u < 0), for the window t-1 .. t-NP.
+  auto yval = [&](int64_t u) -> S {
+    if (u >= 0) return (u < a.T) ? yb[u] : Num<S>::zero();
+    const int64_t k = -u - 1;
+    return (zib && k < a.n) ? zib[k] : Num<S>::zero();
+  };
+  S yr[NP];
+  const int64_t tstart = t0 + a.L - 1;  // first processed (possibly virtual) time
+  if (need_da) {
+#pragma unroll
+    for (int i = 1; i <= NP; ++i) yr[(NP - 1 - i + NP) & (NP - 1)] = Num<S>::conj(yval(tstart - i));
+  }
+  S dacc[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) dacc[i] = Num<S>::zero();
+
+  for (int jb = 0; jb < a.L; jb += NP) {
+#pragma unroll
+    for (int jj = 0; jj < NP; ++jj) {
+      const int j = NP - 1 - jj;  // t mod NP
+      const int64_t t = tstart - jb - jj;
+      if (t < a.T) {
+        S row[NP];
+        load_row<S, NP>(row, a, Ab + t * a.a_tstride);
+        const S lam = gb[t] + ring[j];
+        ring[j] = Num<S>::zero();
+#pragma unroll
+        for (int i = 1; i <= NP; ++i)
+          ring[(j - i) & (NP - 1)] = Num<S>::fms(ring[(j - i) & (NP - 1)], Num<S>::conj(row[i - 1]), lam);
+        if (dxb) dxb[t] = lam;
+        if (need_da) {
+          if (dAb) {
+            S* drow = dAb + t * a.n;
+#pragma unroll
+            for (int i = 1; i <= NP; ++i)
+              if (i <= a.n) drow[i - 1] = coef(a, -(lam * yr[(j - i) & (NP - 1)]));
+          } else {
+#pragma unroll
+            for (int i = 1; i <= NP; ++i) dacc[i - 1] = Num<S>::fms(dacc[i - 1], lam, yr[(j - i) & (NP - 1)]);
+          }
+        }
+      }
+      if (need_da) {
+        // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
+        yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval(t - 1 - NP));
+      }
+    }
+  }
+  if (a.dapart) {
+    S* dp = a.dapart + cid * NP;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) dp[i] = coef(a, dacc[i]);
+  }
+  if (c == 0 && a.dzi) {
+    // pending sums for positions -1-r sit in slots NP-1-r
+#pragma unroll
+    for (int r = 0; r < NP; ++r)
+      if (r < a.n) a.dzi[b * a.n + r] = ring[NP - 1 - r];
+  }
+}
+
+// Time-invariant da: deterministic sum of the per-chunk partials.
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void ti_da_reduce(ChunkArgs<S> a) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.B * a.n) return;
+  const int64_t b = idx / a.n;
+  const int i = (int)(idx - b * a.n);
+  S acc = Num<S>::zero();
+  const S* dp = a.dapart + b * a.P * NP + i;
+  for (int64_t c = 0; c < a.P; ++c) acc = acc + dp[c * NP];
+  a.da_ti[b * a.n + i] = acc;
+}
+
+// ----------------------------------------------------------------------------
+// Host-side launch of the three phases.
+template <typename S, int NP>
+int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
+  using C = ChunkCfg<S, NP>;
+  if (a.P > 1) {
+    const int64_t nsum = a.B * (a.P - 1);
+    const int64_t grid = (nsum + C::G - 1) / C::G;
+    hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
+  }
+  bool blocked = false;
+  if constexpr (blocked_fits<S, NP>()) {
+    if (a.P > 64) {
+      hipLaunchKernelGGL((carry_blocked<S, NP, true>), dim3((unsigned)a.B), dim3(kBlockedThreads), 0, st, a);
+      blocked = true;
+    }
+  }
+  if (!blocked)
+    hipLaunchKernelGGL((carry_serial<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
+  const int64_t nch = a.B * a.P;
+  hipLaunchKernelGGL((fwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+template <typename S, int NP>
+int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
+  using C = ChunkCfg<S, NP>;
+  if (a.P > 1) {
+    const int64_t nsum = a.B * (a.P - 1);
+    const int64_t grid = (nsum + C::G - 1) / C::G;
+    hipLaunchKernelGGL((bwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
+  }
+  bool blocked = false;
+  if constexpr (blocked_fits<S, NP>()) {
+    if (a.P > 64) {
+      hipLaunchKernelGGL((carry_blocked<S, NP, false>), dim3((unsigned)a.B), dim3(kBlockedThreads), 0, st, a);
+      blocked = true;
+    }
+  }
+  if (!blocked)
+    hipLaunchKernelGGL((carry_serial<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
+  const int64_t nch = a.B * a.P;
+  hipLaunchKernelGGL((bwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+  if (a.dapart) {
+    const int64_t nd = a.B * a.n;
+    hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+#define TLPC_INST(S, NP)                                                  \
+  template int launch_chunked_fwd<S, NP>(ChunkArgs<S>, hipStream_t);      \
+  template int launch_chunked_bwd<S, NP>(ChunkArgs<S>, hipStream_t);
+
+#define TLPC_INST_ALL_NP(S) \
+  TLPC_INST(S, 1)           \
+  TLPC_INST(S, 2)           \
+  TLPC_INST(S, 4)           \
+  TLPC_INST(S, 8)           \
+  TLPC_INST(S, 16)          \
+  TLPC_INST(S, 32)
+
+TLPC_INST_ALL_NP(float)
+TLPC_INST_ALL_NP(double)
+TLPC_INST_ALL_NP(Cx<float>)
+TLPC_INST_ALL_NP(Cx<double>)
+
+}  // namespace tlpc

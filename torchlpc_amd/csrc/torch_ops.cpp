@@ -1,0 +1,239 @@
+// PyTorch operator layer over the C ABI (include/torchlpc_amd.h).
+//
+// Registers the namespace `torchlpc_amd` with the ops the Python autograd
+// Functions call. The GPU implementations validate with TORCH_CHECK, allocate the
+// outputs and the workspace from the caching allocator, and launch on the current
+// HIP stream; no call synchronises, so the ops can be captured in a hipGraph.
+// CPU tensors are rejected with an error: the filter runs on the GPU only.
+//
+// Reference op boundary: TORCH_LIBRARY(torchlpc) schemas at
+// torchlpc/csrc/scan_cpu.cpp:166-170 (lpc(x, A, zi), scan(input, weights, initials)).
+#include <ATen/ATen.h>
+#include <c10/core/DeviceGuard.h>
+#include <c10/hip/HIPStream.h>
+#include <torch/library.h>
+
+#include <tuple>
+
+#include "../../include/torchlpc_amd.h"
+
+namespace {
+
+int dtype_code(const at::Tensor& t) {
+  switch (t.scalar_type()) {
+    case at::kFloat: return TLPC_F32;
+    case at::kDouble: return TLPC_F64;
+    case at::kComplexFloat: return TLPC_C64;
+    case at::kComplexDouble: return TLPC_C128;
+    default:
+      TORCH_CHECK(false, "torchlpc_amd: unsupported dtype ", t.scalar_type(),
+                  " (float32, float64, complex64 and complex128 are supported)");
+  }
+  return -1;
+}
+
+void check_rc(int rc, const char* what) {
+  TORCH_CHECK(rc == TLPC_OK, "torchlpc_amd: ", what, " failed: ", tlpc_error_string(rc), " (",
+              rc, ")");
+}
+
+void* stream_of(const at::Tensor& t) {
+  return reinterpret_cast<void*>(c10::hip::getCurrentHIPStream(t.device().index()).stream());
+}
+
+at::Tensor workspace(const at::Tensor& like, size_t bytes) {
+  return at::empty({(int64_t)bytes}, like.options().dtype(at::kByte));
+}
+
+// Coefficient layout resolution. Returns the coefficient tensor to hand to the
+// kernels and the time-invariance flag: a 2-D a[B,N] and a 3-D A whose time
+// stride is 0 (a time broadcast, e.g. a[:, None, :].expand(-1, T, -1)) are read
+// as one row per signal without materialising [B,T,N].
+std::pair<at::Tensor, bool> coef_view(const at::Tensor& A, int64_t T) {
+  if (A.dim() == 2) return {A.contiguous(), true};
+  if (A.size(1) > 1 && A.stride(1) == 0) return {A.select(1, 0).contiguous(), true};
+  return {A.contiguous(), false};
+}
+
+void check_inputs(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi, const char* op) {
+  TORCH_CHECK(x.is_cuda(), "torchlpc_amd::", op,
+              ": tensors must be on a HIP (MI355X) device; CPU execution is not provided");
+  TORCH_CHECK(x.dim() == 2, "torchlpc_amd::", op, ": x must be 2-D [B, T], got ", x.sizes());
+  TORCH_CHECK(A.dim() == 3 || A.dim() == 2, "torchlpc_amd::", op,
+              ": A must be [B, T, N] or [B, N], got ", A.sizes());
+  TORCH_CHECK(A.scalar_type() == x.scalar_type() && zi.scalar_type() == x.scalar_type(),
+              "torchlpc_amd::", op, ": x, A and zi must share a dtype");
+  TORCH_CHECK(A.device() == x.device() && zi.device() == x.device(), "torchlpc_amd::", op,
+              ": x, A and zi must be on the same device");
+  TORCH_CHECK(A.size(0) == x.size(0), "torchlpc_amd::", op, ": batch of A and x differ");
+  if (A.dim() == 3)
+    TORCH_CHECK(A.size(1) == x.size(1), "torchlpc_amd::", op, ": time of A and x differ");
+  const int64_t N = A.size(A.dim() - 1);
+  TORCH_CHECK(zi.dim() == 2 && zi.size(0) == x.size(0) && zi.size(1) == N, "torchlpc_amd::", op,
+              ": zi must be [B, N] = [", x.size(0), ", ", N, "], got ", zi.sizes());
+  TORCH_CHECK(N >= 1, "torchlpc_amd::", op, ": filter order must be >= 1");
+}
+
+at::Tensor lpc_forward_hip(const at::Tensor& x_, const at::Tensor& A_, const at::Tensor& zi_,
+                           int64_t strategy, int64_t chunk_len) {
+  check_inputs(x_, A_, zi_, "lpc");
+  c10::DeviceGuard guard(x_.device());
+  const auto x = x_.contiguous();
+  const auto zi = zi_.contiguous();
+  const int64_t B = x.size(0), T = x.size(1), N = A_.size(A_.dim() - 1);
+  auto [A, ti] = coef_view(A_, T);
+  auto y = at::empty_like(x);
+  const int dt = dtype_code(x);
+  const int flags = ti ? TLPC_TIME_INVARIANT : 0;
+  const size_t wsb = tlpc_lpc_forward_workspace(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
+                                                (int32_t)chunk_len);
+  auto ws = workspace(x, wsb);
+  check_rc(tlpc_lpc_forward(dt, x.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(), B, T,
+                            (int32_t)N, flags, (int32_t)strategy, (int32_t)chunk_len,
+                            wsb ? ws.data_ptr() : nullptr, wsb, stream_of(x)),
+           "lpc forward");
+  return y;
+}
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_hip(
+    const at::Tensor& g_, const at::Tensor& A_, const at::Tensor& zi_, const at::Tensor& y_,
+    bool need_x, bool need_a, bool need_zi, int64_t strategy, int64_t chunk_len) {
+  check_inputs(g_, A_, zi_, "lpc_backward");
+  TORCH_CHECK(y_.sizes() == g_.sizes(), "torchlpc_amd::lpc_backward: y and grad shapes differ");
+  c10::DeviceGuard guard(g_.device());
+  const auto g = g_.contiguous();
+  const auto zi = zi_.contiguous();
+  const auto y = y_.contiguous();
+  const int64_t B = g.size(0), T = g.size(1), N = A_.size(A_.dim() - 1);
+  auto [A, ti] = coef_view(A_, T);
+  const bool full_da = ti && A_.dim() == 3;  // gradient of a time-broadcast A is per sample
+  at::Tensor dx = need_x ? at::empty_like(g) : at::empty({0}, g.options());
+  at::Tensor dA = need_a ? (A_.dim() == 3 ? at::empty({B, T, N}, g.options())
+                                          : at::empty({B, N}, g.options()))
+                         : at::empty({0}, g.options());
+  at::Tensor dzi = need_zi ? at::empty({B, N}, g.options()) : at::empty({0}, g.options());
+  int flags = (need_x ? TLPC_NEED_DX : 0) | (need_a ? TLPC_NEED_DA : 0) |
+              (need_zi ? TLPC_NEED_DZI : 0) | (ti ? TLPC_TIME_INVARIANT : 0) |
+              (full_da ? TLPC_DA_FULL : 0);
+  if (!(need_x || need_a || need_zi)) return {dx, dA, dzi};
+  const int dt = dtype_code(g);
+  const size_t wsb = tlpc_lpc_backward_workspace(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
+                                                 (int32_t)chunk_len);
+  auto ws = workspace(g, wsb);
+  check_rc(tlpc_lpc_backward(dt, g.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(),
+                             need_x ? dx.data_ptr() : nullptr, need_a ? dA.data_ptr() : nullptr,
+                             need_zi ? dzi.data_ptr() : nullptr, B, T, (int32_t)N, flags,
+                             (int32_t)strategy, (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr,
+                             wsb, stream_of(g)),
+           "lpc backward");
+  return {dx, dA, dzi};
+}
+
+void check_scan(const at::Tensor& x, const at::Tensor& w, const at::Tensor& init, const char* op) {
+  TORCH_CHECK(x.is_cuda(), "torchlpc_amd::", op,
+              ": tensors must be on a HIP (MI355X) device; CPU execution is not provided");
+  TORCH_CHECK(x.dim() == 2, "torchlpc_amd::", op, ": input must be 2-D [B, T]");
+  TORCH_CHECK(w.sizes() == x.sizes(), "torchlpc_amd::", op, ": weights must match the input shape");
+  TORCH_CHECK(init.dim() == 1 && init.size(0) == x.size(0), "torchlpc_amd::", op,
+              ": initials must be [B]");
+  TORCH_CHECK(w.scalar_type() == x.scalar_type() && init.scalar_type() == x.scalar_type(),
+              "torchlpc_amd::", op, ": input, weights and initials must share a dtype");
+  TORCH_CHECK(w.device() == x.device() && init.device() == x.device(), "torchlpc_amd::", op,
+              ": tensors must be on the same device");
+}
+
+at::Tensor scan_hip(const at::Tensor& x_, const at::Tensor& w_, const at::Tensor& init_,
+                    int64_t chunk_len) {
+  check_scan(x_, w_, init_, "scan");
+  c10::DeviceGuard guard(x_.device());
+  const auto x = x_.contiguous(), w = w_.contiguous(), init = init_.contiguous();
+  const int64_t B = x.size(0), T = x.size(1);
+  auto h = at::empty_like(x);
+  const int dt = dtype_code(x);
+  const size_t wsb = tlpc_scan_workspace(dt, B, T, 0, (int32_t)chunk_len);
+  auto ws = workspace(x, wsb);
+  check_rc(tlpc_scan(dt, x.data_ptr(), w.data_ptr(), init.data_ptr(), h.data_ptr(), B, T,
+                     (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr, wsb, stream_of(x)),
+           "scan");
+  return h;
+}
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> scan_backward_hip(
+    const at::Tensor& g_, const at::Tensor& w_, const at::Tensor& init_, const at::Tensor& h_,
+    bool need_x, bool need_w, bool need_init, int64_t chunk_len) {
+  check_scan(g_, w_, init_, "scan_backward");
+  c10::DeviceGuard guard(g_.device());
+  const auto g = g_.contiguous(), w = w_.contiguous(), init = init_.contiguous(),
+             h = h_.contiguous();
+  const int64_t B = g.size(0), T = g.size(1);
+  at::Tensor dx = need_x ? at::empty_like(g) : at::empty({0}, g.options());
+  at::Tensor dw = need_w ? at::empty_like(g) : at::empty({0}, g.options());
+  at::Tensor dinit = need_init ? at::empty({B}, g.options()) : at::empty({0}, g.options());
+  if (!(need_x || need_w || need_init)) return {dx, dw, dinit};
+  const int flags = (need_x ? TLPC_NEED_DX : 0) | (need_w ? TLPC_NEED_DA : 0) |
+                    (need_init ? TLPC_NEED_DZI : 0);
+  const int dt = dtype_code(g);
+  const size_t wsb = tlpc_scan_workspace(dt, B, T, flags, (int32_t)chunk_len);
+  auto ws = workspace(g, wsb);
+  check_rc(tlpc_scan_backward(dt, g.data_ptr(), w.data_ptr(), init.data_ptr(), h.data_ptr(),
+                              need_x ? dx.data_ptr() : nullptr, need_w ? dw.data_ptr() : nullptr,
+                              need_init ? dinit.data_ptr() : nullptr, B, T, flags,
+                              (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr, wsb,
+                              stream_of(g)),
+           "scan backward");
+  return {dx, dw, dinit};
+}
+
+[[noreturn]] void cpu_unsupported(const char* op) {
+  TORCH_CHECK(false, "torchlpc_amd::", op,
+              ": CPU tensors are not supported; move x, A and zi to the MI355X device (.cuda())");
+  throw;  // unreachable
+}
+
+at::Tensor lpc_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t, int64_t) {
+  cpu_unsupported("lpc");
+}
+std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_cpu(const at::Tensor&,
+                                                                const at::Tensor&,
+                                                                const at::Tensor&,
+                                                                const at::Tensor&, bool, bool,
+                                                                bool, int64_t, int64_t) {
+  cpu_unsupported("lpc_backward");
+}
+at::Tensor scan_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t) {
+  cpu_unsupported("scan");
+}
+std::tuple<at::Tensor, at::Tensor, at::Tensor> scan_backward_cpu(const at::Tensor&,
+                                                                 const at::Tensor&,
+                                                                 const at::Tensor&,
+                                                                 const at::Tensor&, bool, bool,
+                                                                 bool, int64_t) {
+  cpu_unsupported("scan_backward");
+}
+
+}  // namespace
+
+TORCH_LIBRARY(torchlpc_amd, m) {
+  m.def("lpc(Tensor x, Tensor A, Tensor zi, int strategy=0, int chunk_len=0) -> Tensor");
+  m.def(
+      "lpc_backward(Tensor grad_y, Tensor A, Tensor zi, Tensor y, bool need_x, bool need_a, "
+      "bool need_zi, int strategy=0, int chunk_len=0) -> (Tensor, Tensor, Tensor)");
+  m.def("scan(Tensor x, Tensor w, Tensor init, int chunk_len=0) -> Tensor");
+  m.def(
+      "scan_backward(Tensor grad, Tensor w, Tensor init, Tensor h, bool need_x, bool need_w, "
+      "bool need_init, int chunk_len=0) -> (Tensor, Tensor, Tensor)");
+}
+
+TORCH_LIBRARY_IMPL(torchlpc_amd, CUDA, m) {
+  m.impl("lpc", &lpc_forward_hip);
+  m.impl("lpc_backward", &lpc_backward_hip);
+  m.impl("scan", &scan_hip);
+  m.impl("scan_backward", &scan_backward_hip);
+}
+
+TORCH_LIBRARY_IMPL(torchlpc_amd, CPU, m) {
+  m.impl("lpc", &lpc_cpu);
+  m.impl("lpc_backward", &lpc_backward_cpu);
+  m.impl("scan", &scan_cpu);
+  m.impl("scan_backward", &scan_backward_cpu);
+}
