@@ -16,13 +16,20 @@ REL_TOL = {
 HI = {np.float32: np.float64, np.float64: np.float64, np.complex64: np.complex128, np.complex128: np.complex128}
 
 
-def stable_inputs(rng, B, T, N, dtype, hop=16, kmax=0.8):
-    """x ~ N(0,1), zi ~ N(0,1), stable time-varying A from reflection coefficients."""
+def stable_inputs(rng, B, T, N, dtype, hop=64, kmax=None):
+    """x ~ N(0,1), zi ~ N(0,1), A from reflection coefficients k ~ U(-kmax, kmax) at
+    anchors every `hop` samples, linearly interpolated per sample and converted by
+    the step-up recursion. The default kmax = min(0.8, 4/N) keeps the time-varying
+    filter well conditioned at every order (the FP32 reference loop stays below
+    2e-7 relative error on these inputs), so the bars test the kernels, not the
+    conditioning of the input."""
     cplx = np.issubdtype(dtype, np.complexfloating)
+    if kmax is None:
+        kmax = min(0.8, 4.0 / N)
     x = rng.standard_normal((B, T))
     zi = rng.standard_normal((B, N))
-    nk = T // hop + 2
-    k = np.repeat(rng.uniform(-kmax, kmax, (B, nk, N)), hop, axis=1)[:, :T]
+    nk = (T - 1) // hop + 2
+    k = O._interp_anchors(rng.uniform(-kmax, kmax, (B, nk, N)), T, hop)
     A = O.step_up(k)
     if cplx:
         x = x + 1j * rng.standard_normal((B, T))

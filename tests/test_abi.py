@@ -56,14 +56,16 @@ def test_error_strings(lib):
 
 
 def test_workspace_queries(lib):
-    # chunked: B*(P-1)*(N*N+N) summaries + B*P*N carries (256-byte aligned)
+    # chunked: B*(P-1)*(N*N+N) summaries + B*P*N carries (256-byte aligned), kept in
+    # double precision for float data
     B, T, N = 256, 48000, 16
     ws = lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, N, 0, 1, 512)
     P = -(-T // 512)
-    summ = B * (P - 1) * (N * N + N) * 4
-    carry = B * P * N * 4
+    summ = B * (P - 1) * (N * N + N) * 8
+    carry = B * P * N * 8
     assert ws == -(-summ // 256) * 256 + -(-carry // 256) * 256
-    assert lib.tlpc_lpc_forward_workspace(TLPC_F64, B, T, N, 0, 1, 512) == 2 * ws
+    assert lib.tlpc_lpc_forward_workspace(TLPC_F64, B, T, N, 0, 1, 512) == ws
+    assert lib.tlpc_lpc_forward_workspace(TLPC_C64, B, T, N, 0, 1, 512) == 2 * ws
     # time-invariant reduced da needs per-chunk partials; the full per-sample dA does not
     wb = lib.tlpc_lpc_backward_workspace(TLPC_F32, 8, 4096, 4, NEED_DA | TI, 1, 256)
     wf = lib.tlpc_lpc_backward_workspace(TLPC_F32, 8, 4096, 4, NEED_DA | TI | DA_FULL, 1, 256)

@@ -111,7 +111,11 @@ def test_backward_serial_strategy(dtype):
 def test_time_invariant(dtype, N):
     rng = np.random.default_rng(N)
     B, T = 4, 3000
-    a = O.step_up(rng.uniform(-0.8, 0.8, (B, N)))
+    # k ~ U(-0.8, 0.8) puts poles within 1e-8 of the unit circle at N = 32, where
+    # the serial reference loop itself loses 1e-4 (float32) / 1e-11 (float64) to
+    # rounding; min(0.8, 6/N) keeps the comparison well conditioned.
+    kmax = min(0.8, 6.0 / N)
+    a = O.step_up(rng.uniform(-kmax, kmax, (B, N)))
     x = rng.standard_normal((B, T))
     zi = rng.standard_normal((B, N))
     g = rng.standard_normal((B, T))
@@ -209,3 +213,27 @@ def test_reference_cpu_op_fp32_agreement():
     y64 = O.lpc(x.astype(np.float64), A.astype(np.float64), zi.astype(np.float64))
     assert rel(y, y64) <= 1e-5
     assert rel(y, ref32) <= 1e-5
+
+
+@pytest.mark.parametrize("B,T", [(8, 48000), (2, 2000)])
+def test_headline_inputs_fp32(B, T):
+    """Reflection-coefficient inputs of the B=256, T=48000, N=16 benchmark (k ~ U(-0.9, 0.9)
+    every 240 samples), on a subset of signals: y, dx, dA, dzi within 1e-5 of FP64."""
+    x, A, zi, w = O.make_reflection_inputs(B, T, 16, 240, 1002)
+    x, A, zi, w = (v.astype(np.float32) for v in (x, A, zi, w))
+    yd = ops().lpc(dev(x), dev(A), dev(zi))
+    dx, dA, dzi = ops().lpc_backward(dev(w), dev(A), dev(zi), yd, True, True, True)
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    rx, rA, rzi = O.lpc_backward(up(w, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    errs = [rel(host(yd), y64), rel(host(dx), rx), rel(host(dA), rA), rel(host(dzi), rzi)]
+    assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("T,chunk", [(1, 0), (15, 0), (17, 16), (1000, 16), (3583, 64), (3585, 64), (48000, 0), (20000, 32)])
+def test_signal_strategy_forward(T, chunk):
+    """Per-signal fused kernel (order 16, float32) against the FP64 oracle."""
+    rng = np.random.default_rng(T + chunk)
+    x, A, zi = stable_inputs(rng, 3, T, 16, np.float32, kmax=0.8, hop=240)
+    y = host(ops().lpc(dev(x), dev(A), dev(zi), 2, chunk))
+    ref = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    assert rel(y, ref) <= 1e-5, rel(y, ref)

@@ -22,6 +22,17 @@ inline int compiled_order(int n) {
   return 0;  // served by the serial strategy
 }
 
+// element size of the summaries and carries (float data keeps them in double)
+inline size_t wide_size(int dtype) {
+  switch (dtype) {
+    case TLPC_F32: return 8;
+    case TLPC_F64: return 8;
+    case TLPC_C64: return 16;
+    case TLPC_C128: return 16;
+    default: return 0;
+  }
+}
+
 inline size_t elem_size(int dtype) {
   switch (dtype) {
     case TLPC_F32: return 4;
@@ -33,10 +44,14 @@ inline size_t elem_size(int dtype) {
 }
 
 // Chunk length: a multiple of the staging block, sized so that the rerun phase
-// has about 32K lanes in flight, capped at 4096 samples.
-inline int pick_chunk(int64_t B, int64_t T, int np, int requested) {
+// has about 32K lanes in flight, capped at 4096 samples. Single-precision data
+// keeps chunks at least 16*NP long: every chunk boundary adds the rounding of
+// an NP x NP carry product, and over longer chunks the basis responses have
+// decayed, which keeps the result at the accuracy of a serial recurrence.
+inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool single) {
   const int blk = chunk_block(np);
   int64_t L = requested > 0 ? requested : (B * T + 32767) / 32768;
+  if (requested <= 0 && single && L < 16 * np) L = 16 * np;
   if (L < blk) L = blk;
   if (L > 4096 && requested <= 0) L = 4096;
   if (L > (int64_t)1 << 24) L = (int64_t)1 << 24;
@@ -44,9 +59,17 @@ inline int pick_chunk(int64_t B, int64_t T, int np, int requested) {
   return (int)L;
 }
 
-inline int pick_strategy(int strategy, int np) {
+// The per-signal fused kernel exists for the float32, order-16, time-varying
+// forward; it needs enough signals to occupy the chip (one workgroup each).
+inline bool signal_ok(int dtype, int64_t B, int n, int flags, bool backward, int neg) {
+  return dtype == TLPC_F32 && n == 16 && !(flags & TLPC_TIME_INVARIANT) && !backward && !neg &&
+         B >= 1;
+}
+
+inline int pick_strategy(int strategy, int np, bool sig_ok, int64_t B) {
   if (np == 0) return TLPC_STRATEGY_SERIAL;
   if (strategy == TLPC_STRATEGY_SERIAL) return TLPC_STRATEGY_SERIAL;
+  if (strategy == TLPC_STRATEGY_SIGNAL && sig_ok) return TLPC_STRATEGY_SIGNAL;
   return TLPC_STRATEGY_CHUNKED;
 }
 
@@ -59,21 +82,27 @@ struct Plan {
 };
 
 Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
-               int32_t chunk_len, bool backward) {
+               int32_t chunk_len, bool backward, int neg = 0) {
   Plan p;
   const size_t es = elem_size(dtype);
   p.np = compiled_order(n);
-  p.strategy = pick_strategy(strategy, p.np);
+  p.strategy = pick_strategy(strategy, p.np, signal_ok(dtype, B, n, flags, backward, neg), B);
+  if (p.strategy == TLPC_STRATEGY_SIGNAL) {
+    p.L = signal_chunk_len(T, chunk_len);
+    p.P = 1;
+    return p;  // no workspace
+  }
   if (p.strategy == TLPC_STRATEGY_SERIAL) {
     // the serial adjoint keeps lambda in the workspace when dx is not requested
     if (backward && !(flags & TLPC_NEED_DX)) p.total = align_up((size_t)B * T * es);
     p.carry_off = 0;
     return p;
   }
-  p.L = pick_chunk(B, T, p.np, chunk_len);
+  p.L = pick_chunk(B, T, p.np, chunk_len, dtype == TLPC_F32 || dtype == TLPC_C64);
   p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
-  const size_t summ = (size_t)B * (p.P > 1 ? p.P - 1 : 0) * (p.np * p.np + p.np) * es;
-  const size_t carry = (size_t)B * p.P * p.np * es;
+  const size_t ws_es = wide_size(dtype);
+  const size_t summ = (size_t)B * (p.P > 1 ? p.P - 1 : 0) * (p.np * p.np + p.np) * ws_es;
+  const size_t carry = (size_t)B * p.P * p.np * ws_es;
   const size_t dapart =
       (backward && (flags & TLPC_TIME_INVARIANT) && (flags & TLPC_NEED_DA) && !(flags & TLPC_DA_FULL))
           ? (size_t)B * p.P * p.np * es
@@ -115,7 +144,7 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
         void* dzi, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
         int32_t chunk_len, void* ws, size_t ws_bytes, void* stream, bool backward, int neg) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const Plan p = make_plan(dtype, B, T, n, flags, strategy, chunk_len, backward);
+  const Plan p = make_plan(dtype, B, T, n, flags, strategy, chunk_len, backward, neg);
   if (B == 0) return TLPC_OK;
   const bool ti = (flags & TLPC_TIME_INVARIANT) != 0;
   const bool da_reduced = ti && !(flags & TLPC_DA_FULL);
@@ -155,16 +184,30 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   if (p.strategy == TLPC_STRATEGY_SERIAL) {
     a.dA = (need_da && !da_reduced) ? static_cast<S*>(dA) : nullptr;
     a.da_ti = (need_da && da_reduced) ? static_cast<S*>(dA) : nullptr;
-    a.carry = reinterpret_cast<S*>(w);
+    a.carry = reinterpret_cast<typename Wide<S>::type*>(w);
     return backward ? launch_serial_bwd<S>(a, st) : launch_serial_fwd<S>(a, st);
   }
-  a.summ = reinterpret_cast<S*>(w + p.summ_off);
-  a.carry = reinterpret_cast<S*>(w + p.carry_off);
+  if (p.strategy == TLPC_STRATEGY_SIGNAL) {
+    if constexpr (sizeof(S) == 4) return launch_signal_fwd16(a, st);
+    return TLPC_EINVAL;
+  }
+  using W = typename Wide<S>::type;
+  a.summ = reinterpret_cast<W*>(w + p.summ_off);
+  a.carry = reinterpret_cast<W*>(w + p.carry_off);
   if (need_da && da_reduced) {
     a.dapart = reinterpret_cast<S*>(w + p.dapart_off);
     a.da_ti = static_cast<S*>(dA);
   } else if (need_da) {
     a.dA = static_cast<S*>(dA);
+  }
+  if constexpr (sizeof(S) == 4) {
+    // float32, order 16, time-varying: DPP summaries and quad reruns
+    if (n == 16 && !ti && !neg && p.L % 16 == 0) {
+      int rc = backward ? launch_chunk16_adj_summary(a, st) : launch_chunk16_fwd_summary(a, st);
+      if (rc == TLPC_OK) rc = launch_chunk16_carry(a, !backward, st);
+      if (rc == TLPC_OK) rc = backward ? launch_chunk16_adj_rerun(a, st) : launch_chunk16_fwd_rerun(a, st);
+      return rc;
+    }
   }
   return dispatch_np<S>(p, a, backward, st);
 }

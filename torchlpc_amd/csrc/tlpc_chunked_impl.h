@@ -20,6 +20,8 @@
 // for lags n < i <= NP are zero, so padded lags never change a result. The state
 // ring is indexed statically: time t lives in slot t mod NP, every chunk starts
 // at a multiple of NP, and the unrolled inner loop covers NP steps.
+#pragma once
+
 #include "tlpc_common.h"
 #include "tlpc_kernels.h"
 
@@ -30,10 +32,11 @@ struct ChunkCfg {
   static constexpr int TB = NP < 16 ? 16 : NP;  // time steps staged per LDS block
   static constexpr int LDS_BUDGET = 40 * 1024;
   static constexpr int G_LANES = 256 / (NP + 1);
-  static constexpr int G_LDS = LDS_BUDGET / (TB * (NP + 1) * (int)sizeof(S));
+  using W = typename Wide<S>::type;  // summaries and carries
+  static constexpr int G_LDS = LDS_BUDGET / (TB * (NP + 1) * (int)sizeof(W));
   static constexpr int G = (G_LANES < G_LDS ? G_LANES : G_LDS) < 1 ? 1 : (G_LANES < G_LDS ? G_LANES : G_LDS);
   static constexpr int THREADS = ((G * (NP + 1) + 63) / 64) * 64;
-  static constexpr int ASTRIDE = TB * NP + (16 / (int)sizeof(S) > 0 ? 16 / (int)sizeof(S) : 1);
+  static constexpr int ASTRIDE = TB * NP + (16 / (int)sizeof(W) > 0 ? 16 / (int)sizeof(W) : 1);
   static constexpr int SUMM = NP * NP + NP;  // elements per chunk summary
 };
 
@@ -42,10 +45,12 @@ struct ChunkCfg {
 // time block [t0, t0 + TB) into LDS; rows beyond the signal end and lags >= n
 // are zero.
 template <typename S, int NP>
-__device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, S* sA, S* sX, const S* inp,
+__device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide<S>::type* sA,
+                                            typename Wide<S>::type* sX, const S* inp,
                                             int64_t first, int64_t nchunks, int64_t cpsig,
                                             int64_t c_base, int t0) {
   using C = ChunkCfg<S, NP>;
+  using W = typename Wide<S>::type;
   const int nthr = blockDim.x;
   for (int idx = threadIdx.x; idx < C::G * C::TB * NP; idx += nthr) {
     const int gg = idx / (C::TB * NP);
@@ -53,12 +58,12 @@ __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, S* sA, S* sX,
     const int tb = r / NP;
     const int i = r - tb * NP;
     const int64_t cid = first + gg;
-    S v = Num<S>::zero();
+    W v = Num<W>::zero();
     if (cid < nchunks && i < a.n) {
       const int64_t b = cid / cpsig;
       const int64_t c = cid - b * cpsig + c_base;
       const int64_t t = c * a.L + t0 + tb;
-      if (t < a.T) v = coef(a, a.A[b * a.a_bstride + t * a.a_tstride + i]);
+      if (t < a.T) v = convert<W>(coef(a, a.A[b * a.a_bstride + t * a.a_tstride + i]));
     }
     sA[gg * C::ASTRIDE + tb * NP + i] = v;
   }
@@ -66,12 +71,12 @@ __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, S* sA, S* sX,
     const int gg = idx / C::TB;
     const int tb = idx - gg * C::TB;
     const int64_t cid = first + gg;
-    S v = Num<S>::zero();
+    W v = Num<W>::zero();
     if (cid < nchunks) {
       const int64_t b = cid / cpsig;
       const int64_t c = cid - b * cpsig + c_base;
       const int64_t t = c * a.L + t0 + tb;
-      if (t < a.T) v = inp[b * a.T + t];
+      if (t < a.T) v = convert<W>(inp[b * a.T + t]);
     }
     sX[gg * C::TB + tb] = v;
   }
@@ -82,8 +87,9 @@ __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, S* sA, S* sX,
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
   using C = ChunkCfg<S, NP>;
-  __shared__ S sA[C::G * C::ASTRIDE];
-  __shared__ S sX[C::G * C::TB];
+  using W = typename Wide<S>::type;
+  __shared__ W sA[C::G * C::ASTRIDE];
+  __shared__ W sX[C::G * C::TB];
   const int64_t cpsig = a.P - 1;
   const int64_t nchunks = a.B * cpsig;
   const int64_t first = (int64_t)blockIdx.x * C::G;
@@ -91,33 +97,33 @@ __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
   const int k = threadIdx.x - g * (NP + 1);  // k < NP: basis e_k; k == NP: zero state
   const bool active = g < C::G && first + g < nchunks;
 
-  S ring[NP];
+  W ring[NP];
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[r] = Num<S>::zero();
+  for (int r = 0; r < NP; ++r) ring[r] = Num<W>::zero();
   // s_in = e_k means y_{-1-k} = 1, which lives in slot (-1-k) mod NP.
 #pragma unroll
   for (int r = 0; r < NP; ++r)
-    if (k == r) ring[NP - 1 - r] = Num<S>::one();
-  const S xmask = (k == NP) ? Num<S>::one() : Num<S>::zero();
+    if (k == r) ring[NP - 1 - r] = Num<W>::one();
+  const W xmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
 
   for (int t0 = 0; t0 < a.L; t0 += C::TB) {
     stage_block<S, NP>(a, sA, sX, a.x, first, nchunks, cpsig, 0, t0);
     __syncthreads();
     if (active) {
-      const S* rowbase = sA + g * C::ASTRIDE;
+      const W* rowbase = sA + g * C::ASTRIDE;
 #pragma unroll
       for (int j = 0; j < C::TB; ++j) {
-        const S* row = rowbase + j * NP;
-        S acc = sX[g * C::TB + j] * xmask;
+        const W* row = rowbase + j * NP;
+        W acc = sX[g * C::TB + j] * xmask;
 #pragma unroll
-        for (int i = NP; i >= 1; --i) acc = Num<S>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
+        for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
         ring[j & (NP - 1)] = acc;
       }
     }
     __syncthreads();
   }
   if (active) {
-    S* out = a.summ + (first + g) * C::SUMM;
+    W* out = a.summ + (first + g) * C::SUMM;
     // final state s[r] = y_{L-1-r}, slot (L-1-r) mod NP = NP-1-r
     if (k < NP) {
 #pragma unroll
@@ -133,8 +139,9 @@ __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   using C = ChunkCfg<S, NP>;
-  __shared__ S sA[C::G * C::ASTRIDE];
-  __shared__ S sX[C::G * C::TB];
+  using W = typename Wide<S>::type;
+  __shared__ W sA[C::G * C::ASTRIDE];
+  __shared__ W sX[C::G * C::TB];
   const int64_t cpsig = a.P - 1;
   const int64_t nchunks = a.B * cpsig;
   const int64_t first = (int64_t)blockIdx.x * C::G;
@@ -142,36 +149,36 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   const int k = threadIdx.x - g * (NP + 1);
   const bool active = g < C::G && first + g < nchunks;
 
-  S ring[NP];
+  W ring[NP];
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[r] = Num<S>::zero();
+  for (int r = 0; r < NP; ++r) ring[r] = Num<W>::zero();
   // p_in = e_k: pending sum for position t1-1-k, slot (t1-1-k) mod NP = NP-1-k.
 #pragma unroll
   for (int r = 0; r < NP; ++r)
-    if (k == r) ring[NP - 1 - r] = Num<S>::one();
-  const S gmask = (k == NP) ? Num<S>::one() : Num<S>::zero();
+    if (k == r) ring[NP - 1 - r] = Num<W>::one();
+  const W gmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
 
   for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
     stage_block<S, NP>(a, sA, sX, a.x, first, nchunks, cpsig, 1, t0);
     __syncthreads();
     if (active) {
-      const S* rowbase = sA + g * C::ASTRIDE;
+      const W* rowbase = sA + g * C::ASTRIDE;
 #pragma unroll
       for (int jj = 0; jj < C::TB; ++jj) {
         const int j = C::TB - 1 - jj;  // local time; slot of position t is j mod NP
-        const S* row = rowbase + j * NP;
+        const W* row = rowbase + j * NP;
         const int h = j & (NP - 1);
-        const S lam = sX[g * C::TB + j] * gmask + ring[h];
-        ring[h] = Num<S>::zero();
+        const W lam = sX[g * C::TB + j] * gmask + ring[h];
+        ring[h] = Num<W>::zero();
 #pragma unroll
         for (int i = 1; i <= NP; ++i)
-          ring[(j - i) & (NP - 1)] = Num<S>::fms(ring[(j - i) & (NP - 1)], Num<S>::conj(row[i - 1]), lam);
+          ring[(j - i) & (NP - 1)] = Num<W>::fms(ring[(j - i) & (NP - 1)], Num<W>::conj(row[i - 1]), lam);
       }
     }
     __syncthreads();
   }
   if (active) {
-    S* out = a.summ + (first + g) * C::SUMM;
+    W* out = a.summ + (first + g) * C::SUMM;
     // final pending for position t0-1-r sits in slot NP-1-r
     if (k < NP) {
 #pragma unroll
@@ -194,21 +201,22 @@ __global__ __launch_bounds__(64) void carry_serial(ChunkArgs<S> a) {
   const int r = threadIdx.x;
   const int64_t P = a.P;
   const int64_t cpsig = P - 1;
-  S s = Num<S>::zero();
-  if (FWD && r < a.n && a.zi) s = a.zi[b * a.n + r];
-  S* carry = a.carry + b * P * NP;
-  const S* summ = a.summ + b * cpsig * ChunkCfg<S, NP>::SUMM;
+  using W = typename Wide<S>::type;
+  W s = Num<W>::zero();
+  if (FWD && r < a.n && a.zi) s = convert<W>(a.zi[b * a.n + r]);
+  W* carry = a.carry + b * P * NP;
+  const W* summ = a.summ + b * cpsig * ChunkCfg<S, NP>::SUMM;
   for (int64_t step = 0; step < P; ++step) {
     const int64_t c = FWD ? step : P - 1 - step;
     if (r < NP) carry[c * NP + r] = s;
     if (step == P - 1) break;
     // summary index: forward chunk c is at c; adjoint chunk c (>=1) is at c-1
-    const S* sm = summ + (FWD ? c : c - 1) * ChunkCfg<S, NP>::SUMM;
-    S acc = (r < NP) ? sm[NP * NP + r] : Num<S>::zero();
+    const W* sm = summ + (FWD ? c : c - 1) * ChunkCfg<S, NP>::SUMM;
+    W acc = (r < NP) ? sm[NP * NP + r] : Num<W>::zero();
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      const S sk = shfl(s, k);
-      if (r < NP) acc = Num<S>::fma(acc, sm[r * NP + k], sk);
+      const W sk = shfl(s, k);
+      if (r < NP) acc = Num<W>::fma(acc, sm[r * NP + k], sk);
     }
     s = acc;
   }
@@ -268,7 +276,8 @@ constexpr int kBlockedThreads = 256;
 
 template <typename S, int NP, bool FWD>
 __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a) {
-  __shared__ Affine<S, NP> sh[kBlockedThreads];
+  using W = typename Wide<S>::type;
+  __shared__ Affine<W, NP> sh[kBlockedThreads];
   const int64_t b = blockIdx.x;
   const int tid = threadIdx.x;
   const int64_t P = a.P;
@@ -276,13 +285,13 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
   const int64_t per = (links + kBlockedThreads - 1) / kBlockedThreads;
   const int64_t lo = tid * per;
   const int64_t hi = (lo + per < links) ? lo + per : links;
-  const S* summ = a.summ + b * links * ChunkCfg<S, NP>::SUMM;
+  const W* summ = a.summ + b * links * ChunkCfg<S, NP>::SUMM;
   // link l (0-based, in processing order) maps carry of step l to carry of step l+1;
   // forward: chunk c = l at summary l; adjoint: chunk c = P-1-l at summary P-2-l.
-  Affine<S, NP> f;
+  Affine<W, NP> f;
   affine_identity(f);
   for (int64_t l = lo; l < hi; ++l) {
-    Affine<S, NP> m;
+    Affine<W, NP> m;
     load_affine(m, summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM);
     f = affine_compose(m, f);
   }
@@ -290,7 +299,7 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
   __syncthreads();
   // inclusive Hillis-Steele scan over threads (composition in thread order)
   for (int off = 1; off < kBlockedThreads; off <<= 1) {
-    Affine<S, NP> other;
+    Affine<W, NP> other;
     const bool has = tid >= off;
     if (has) other = sh[tid - off];
     __syncthreads();
@@ -301,34 +310,35 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
     __syncthreads();
   }
   // exclusive prefix applied to the initial carry
-  S s[NP];
+  W s[NP];
 #pragma unroll
-  for (int r = 0; r < NP; ++r) s[r] = (FWD && r < a.n && a.zi) ? a.zi[b * a.n + r] : Num<S>::zero();
+  for (int r = 0; r < NP; ++r)
+    s[r] = (FWD && r < a.n && a.zi) ? convert<W>(a.zi[b * a.n + r]) : Num<W>::zero();
   if (tid > 0) {
-    const Affine<S, NP>& pre = sh[tid - 1];
-    S t[NP];
+    const Affine<W, NP>& pre = sh[tid - 1];
+    W t[NP];
 #pragma unroll
     for (int r = 0; r < NP; ++r) {
-      S acc = pre.v[r];
+      W acc = pre.v[r];
 #pragma unroll
-      for (int k = 0; k < NP; ++k) acc = Num<S>::fma(acc, pre.M[r][k], s[k]);
+      for (int k = 0; k < NP; ++k) acc = Num<W>::fma(acc, pre.M[r][k], s[k]);
       t[r] = acc;
     }
 #pragma unroll
     for (int r = 0; r < NP; ++r) s[r] = t[r];
   }
-  S* carry = a.carry + b * P * NP;
+  W* carry = a.carry + b * P * NP;
   for (int64_t l = lo; l < hi; ++l) {
     const int64_t c = FWD ? l : P - 1 - l;
 #pragma unroll
     for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
-    const S* sm = summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM;
-    S t[NP];
+    const W* sm = summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM;
+    W t[NP];
 #pragma unroll
     for (int r = 0; r < NP; ++r) {
-      S acc = sm[NP * NP + r];
+      W acc = sm[NP * NP + r];
 #pragma unroll
-      for (int k = 0; k < NP; ++k) acc = Num<S>::fma(acc, sm[r * NP + k], s[k]);
+      for (int k = 0; k < NP; ++k) acc = Num<W>::fma(acc, sm[r * NP + k], s[k]);
       t[r] = acc;
     }
 #pragma unroll
@@ -346,7 +356,7 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
 // fits in 48 KiB.
 template <typename S, int NP>
 constexpr bool blocked_fits() {
-  return (size_t)NP * (NP + 1) * sizeof(S) * kBlockedThreads <= 48 * 1024;
+  return (size_t)NP * (NP + 1) * sizeof(typename Wide<S>::type) * kBlockedThreads <= 48 * 1024;
 }
 
 // ----------------------------------------------------------------------------
@@ -373,9 +383,9 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   const int64_t rem = a.T - t0;
   const int len = rem < a.L ? (int)rem : a.L;
   S ring[NP];
-  const S* cr = a.carry + cid * NP;
+  const typename Wide<S>::type* cr = a.carry + cid * NP;
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = convert<S>(cr[r]);
   const S* xb = a.x + b * a.T;
   const S* Ab = a.A + b * a.a_bstride;
   S* yb = a.y + b * a.T;
@@ -407,9 +417,9 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   const int64_t c = cid - b * a.P;
   const int64_t t0 = c * a.L;
   S ring[NP];  // pending sums, slot u mod NP for position u
-  const S* cr = a.carry + cid * NP;
+  const typename Wide<S>::type* cr = a.carry + cid * NP;
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = convert<S>(cr[r]);
   const S* gb = a.x + b * a.T;  // upstream gradient
   const S* Ab = a.A + b * a.a_bstride;
   const S* yb = a.y + b * a.T;
@@ -541,8 +551,8 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-#define TLPC_INST(S, NP)                                                  \
-  template int launch_chunked_fwd<S, NP>(ChunkArgs<S>, hipStream_t);      \
+#define TLPC_INST(S, NP)                                             \
+  template int launch_chunked_fwd<S, NP>(ChunkArgs<S>, hipStream_t); \
   template int launch_chunked_bwd<S, NP>(ChunkArgs<S>, hipStream_t);
 
 #define TLPC_INST_ALL_NP(S) \
@@ -552,10 +562,5 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   TLPC_INST(S, 8)           \
   TLPC_INST(S, 16)          \
   TLPC_INST(S, 32)
-
-TLPC_INST_ALL_NP(float)
-TLPC_INST_ALL_NP(double)
-TLPC_INST_ALL_NP(Cx<float>)
-TLPC_INST_ALL_NP(Cx<double>)
 
 }  // namespace tlpc

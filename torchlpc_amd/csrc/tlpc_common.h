@@ -46,6 +46,43 @@ struct Num<Cx<R>> {
   }
 };
 
+// Wide type used for chunk summaries and carries: the basis responses of a chunk
+// accumulate rounding over the whole chunk, so single-precision data keeps them
+// in double precision; double-precision data stays as it is.
+template <typename S>
+struct Wide {
+  using type = S;
+};
+template <>
+struct Wide<float> {
+  using type = double;
+};
+template <>
+struct Wide<Cx<float>> {
+  using type = Cx<double>;
+};
+
+template <typename D, typename S>
+__device__ __forceinline__ D convert(S v) {
+  return (D)v;
+}
+template <>
+__device__ __forceinline__ Cx<double> convert<Cx<double>, Cx<float>>(Cx<float> v) {
+  return {(double)v.re, (double)v.im};
+}
+template <>
+__device__ __forceinline__ Cx<float> convert<Cx<float>, Cx<double>>(Cx<double> v) {
+  return {(float)v.re, (float)v.im};
+}
+template <>
+__device__ __forceinline__ Cx<float> convert<Cx<float>, Cx<float>>(Cx<float> v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ Cx<double> convert<Cx<double>, Cx<double>>(Cx<double> v) {
+  return v;
+}
+
 // Lane shuffle of a scalar (real or complex) within a wavefront.
 __device__ __forceinline__ float shfl(float v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }

@@ -1,0 +1,181 @@
+// Order-16 building blocks on CDNA4 data-parallel primitives (DPP), shared by
+// the per-signal kernel and the order-16 chunked-scan kernels.
+//
+// Matrix rows: a 16-lane DPP row advances 16 independent float64 recurrences
+// (one basis vector per lane) through one sub-chunk. Lane i of the row holds
+// coefficient A[t, i+1] of the current step and every lane reads it through a
+// row_newbcast operand of v_fmac_f64: one instruction per lag, no LDS traffic.
+//
+// Quads: four lanes share one serial recurrence; lane ql owns lags 4ql+1..4ql+4
+// (a float4 of the A row per step, so a quad loads a whole 64-byte row) and a
+// 4-slot window of the state; partial sums meet through quad_perm DPP.
+#pragma once
+
+#include "tlpc_common.h"
+
+namespace tlpc {
+namespace dpp16 {
+
+constexpr int NP = 16;
+constexpr int PF = 8;  // rows in flight per quad lane
+
+// ---------------------------------------------------------------------------
+// Forward matrix step (slot J = t mod 16 of the output ring r, r[u mod 16] = y_u):
+//   returns -sum_{i=1..16} A[t,i] * r[(J - i) & 15]
+// Two accumulators interleave even and odd lags. One asm statement, opened by
+// s_nop 1: the two wait states a DPP source needs after the VALU conversion
+// that produced it.
+template <int J>
+__device__ __forceinline__ double fwd_row_step(double a, const double (&r)[NP]) {
+  double acc0 = 0.0, acc1 = 0.0;
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, -%2, %3 row_newbcast:15 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %4 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %5 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %6 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %7 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %8 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %9 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %14 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %15 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %16 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%2, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%2, %18 row_newbcast:0 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc0), "+v"(acc1)
+      : "v"(a), "v"(r[(J - 16) & 15]), "v"(r[(J - 15) & 15]), "v"(r[(J - 14) & 15]),
+        "v"(r[(J - 13) & 15]), "v"(r[(J - 12) & 15]), "v"(r[(J - 11) & 15]),
+        "v"(r[(J - 10) & 15]), "v"(r[(J - 9) & 15]), "v"(r[(J - 8) & 15]), "v"(r[(J - 7) & 15]),
+        "v"(r[(J - 6) & 15]), "v"(r[(J - 5) & 15]), "v"(r[(J - 4) & 15]), "v"(r[(J - 3) & 15]),
+        "v"(r[(J - 2) & 15]), "v"(r[(J - 1) & 15]));
+  return acc0 + acc1;
+}
+
+// Adjoint (transposed) matrix step at slot J = t mod 16 of the pending ring p
+// (p[u mod 16] = pending sum for position u, positions t .. t-15):
+//   lam = p[J];  p[J] = 0;  p[(J - i) & 15] -= A[t,i] * lam  (i = 1..16)
+// Lag 1 goes first: it feeds the next step's lam. For complex data the caller
+// passes conj(A); here the data is real.
+template <int J>
+__device__ __forceinline__ void adj_row_step(double a, double (&p)[NP]) {
+  const double lam = p[J];
+  p[J] = 0.0;
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, -%16, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, -%16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, -%16, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, -%16, %17 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, -%16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, -%16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %7, -%16, %17 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %8, -%16, %17 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %9, -%16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %10, -%16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %11, -%16, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %12, -%16, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %13, -%16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %14, -%16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %15, -%16, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf"
+      : "+v"(p[(J - 1) & 15]), "+v"(p[(J - 2) & 15]), "+v"(p[(J - 3) & 15]),
+        "+v"(p[(J - 4) & 15]), "+v"(p[(J - 5) & 15]), "+v"(p[(J - 6) & 15]),
+        "+v"(p[(J - 7) & 15]), "+v"(p[(J - 8) & 15]), "+v"(p[(J - 9) & 15]),
+        "+v"(p[(J - 10) & 15]), "+v"(p[(J - 11) & 15]), "+v"(p[(J - 12) & 15]),
+        "+v"(p[(J - 13) & 15]), "+v"(p[(J - 14) & 15]), "+v"(p[(J - 15) & 15]),
+        "+v"(p[(J - 16) & 15])
+      : "v"(a), "v"(lam));
+}
+
+// Carry link on lanes 0-15 (lane r owns s_r): returns z_r + sum_k M[r][k] s_k in
+// float64, with s_k reaching every lane through a row_newbcast operand. s comes
+// from the VALU at the end of the previous link, hence the s_nop 1.
+__device__ __forceinline__ double chain_link(double s, const double (&m)[NP], double z) {
+  double acc0 = z, acc1 = 0.0;
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %2, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %7 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %8 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %9 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %10 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %11 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %12 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %13 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %14 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %15 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %16 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %2, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %2, %18 row_newbcast:15 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc0), "+v"(acc1)
+      : "v"(s), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]),
+        "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]),
+        "v"(m[14]), "v"(m[15]));
+  return acc0 + acc1;
+}
+
+// ---------------------------------------------------------------------------
+// Quad helpers.
+template <int CTRL>
+__device__ __forceinline__ float qperm(float v) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double qperm(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffll), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int QP_SWAP1 = 0xB1;  // quad_perm:[1,0,3,2]
+constexpr int QP_SWAP2 = 0x4E;  // quad_perm:[2,3,0,1]
+constexpr int QP_UP = 0x90;     // quad_perm:[0,0,1,2]  lane ql reads lane ql-1
+constexpr int QP_DOWN = 0xF9;   // quad_perm:[1,2,3,3]  lane ql reads lane ql+1
+constexpr int QP_LANE0 = 0x00;  // quad_perm:[0,0,0,0]
+
+template <typename V>
+__device__ __forceinline__ V quad_sum(V p) {
+  p += qperm<QP_SWAP1>(p);
+  p += qperm<QP_SWAP2>(p);
+  return p;
+}
+
+// Forward quad step. Lane ql's window R holds y_u for u in [t-4-4ql, t-1-4ql] at
+// slot u mod 4; c = A[t, 4ql .. 4ql+3]. Returns y_t (in every lane of the quad).
+template <int J, typename V>
+__device__ __forceinline__ V fwd_quad_step(V (&R)[4], const float4 c, float x, int ql) {
+  const float cc[4] = {c.x, c.y, c.z, c.w};
+  V p = (V)cc[(J - 1) & 3] * R[0];
+  p += (V)cc[(J - 2) & 3] * R[1];
+  p += (V)cc[(J - 3) & 3] * R[2];
+  p += (V)cc[(J - 4) & 3] * R[3];
+  const V y = (V)x - quad_sum(p);
+  const V up = qperm<QP_UP>(R[J]);  // lane ql-1 holds y_{t-4ql} in slot J
+  R[J] = (ql == 0) ? y : up;
+  return y;
+}
+
+// Adjoint quad step at time t (slot J = t mod 4), descending t. State: `now` in
+// lane 0 is the pending sum of position t; lane ql's window P holds the pending
+// sums of positions [t-4-4ql, t-1-4ql] at slot u mod 4. Returns lambda_t and
+// leaves the state for step t-1.
+template <int J, typename V>
+__device__ __forceinline__ V adj_quad_step(V (&P)[4], V& now, const float4 c, float g, int ql) {
+  const V lam = qperm<QP_LANE0>((V)g + now);
+  const float cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) P[(J - 1 - m) & 3] -= (V)cc[m] * lam;
+  constexpr int S = (J - 1) & 3;  // slot that leaves at the top and enters at the bottom
+  const V down = qperm<QP_DOWN>(P[S]);
+  now = P[S];  // meaningful in lane 0: pending sum of position t-1
+  P[S] = (ql == 3) ? (V)0 : down;
+  return lam;
+}
+
+}  // namespace dpp16
+}  // namespace tlpc

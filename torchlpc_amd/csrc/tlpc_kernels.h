@@ -27,8 +27,8 @@ struct ChunkArgs {
   int64_t P;  // chunks per signal
   int64_t a_bstride, a_tstride;  // element strides of A over signals and time (0: time-invariant)
   int neg;    // 1: the coefficient array holds -A (first-order scan weights w = -A)
-  S* summ;    // chunk summaries [B, P-1, NP*NP+NP]
-  S* carry;   // chunk carries [B, P, NP]
+  typename Wide<S>::type* summ;   // chunk summaries [B, P-1, NP*NP+NP]
+  typename Wide<S>::type* carry;  // chunk carries [B, P, NP]
 };
 
 template <typename S, int NP>
@@ -43,6 +43,17 @@ int launch_serial_bwd(ChunkArgs<S> a, hipStream_t st);
 
 template <typename S>
 int launch_fill_zero(S* p, int64_t count, hipStream_t st);
+
+// Order-16 float32 chunked-scan kernels (DPP matrix rows, quad reruns).
+int launch_chunk16_fwd_summary(ChunkArgs<float> a, hipStream_t st);
+int launch_chunk16_adj_summary(ChunkArgs<float> a, hipStream_t st);
+int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st);
+int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st);
+int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st);
+
+// Per-signal fused kernels (order 16, float32): one workgroup per signal.
+int launch_signal_fwd16(ChunkArgs<float> a, hipStream_t st);
+int signal_chunk_len(int64_t T, int requested);
 
 // Coefficient as used by the recurrence: A, or -w for the first-order scan.
 template <typename S>

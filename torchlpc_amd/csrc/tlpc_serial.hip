@@ -86,7 +86,8 @@ int launch_serial_fwd(ChunkArgs<S> a, hipStream_t st) {
 
 template <typename S>
 int launch_serial_bwd(ChunkArgs<S> a, hipStream_t st) {
-  S* lam = a.dx ? a.dx : a.carry;  // the host sizes `carry` to [B,T] for this strategy
+  // lambda lives in dx, or in the workspace (`carry`, sized [B,T] for this strategy)
+  S* lam = a.dx ? a.dx : reinterpret_cast<S*>(a.carry);
   hipLaunchKernelGGL((bwd_serial<S>), dim3((unsigned)((a.B + 63) / 64)), dim3(64), 0, st, a, lam);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
