@@ -156,13 +156,19 @@ def main():
     n = args.steps
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(n)]
 
+    # the step is what LPC.apply + backward run: the forward keeps its workspace
+    # (chunk transition matrices) and the backward, with the forward's strategy
+    # and chunk length, reuses it
     def step(e=None):
         if e:
             e[0].record(stream)
-        y = ops.lpc(x, A, zi, s_f, L_f)
+        y, ws = ops.lpc_keep(x, A, zi, s_f, L_f)
         if e:
             e[1].record(stream)
-        out = ops.lpc_backward(w, A, zi, y, True, True, True, s_b, L_b)
+        if ws.numel() > 0:
+            out = ops.lpc_backward(w, A, zi, y, True, True, True, s_f, L_f, ws)
+        else:
+            out = ops.lpc_backward(w, A, zi, y, True, True, True, s_b, L_b)
         if e:
             e[2].record(stream)
         return y, out

@@ -53,6 +53,12 @@ extern "C" {
 /* with TLPC_TIME_INVARIANT in the backward: write the per-sample dA[B,T,N]
    (the gradient of a time-broadcast A) instead of the reduced da[B,N] */
 #define TLPC_DA_FULL 16
+/* backward: the first tlpc_lpc_summary_bytes() bytes of the workspace still hold
+   what tlpc_lpc_forward wrote there for the same dtype, B, T, N, TIME_INVARIANT
+   flag, strategy and chunk_len (the chunk transition matrices depend on A only);
+   the backward reads them instead of recomputing them. Ignored when
+   tlpc_lpc_summary_bytes() is 0. */
+#define TLPC_REUSE_SUMMARY 32
 
 /* strategy for tlpc_lpc_forward / tlpc_lpc_backward (`strategy` argument) */
 #define TLPC_STRATEGY_AUTO 0    /* library heuristic */
@@ -90,6 +96,17 @@ int tlpc_lpc_backward(int dtype, const void* g, const void* A, const void* zi, c
                       void* dx, void* dA, void* dzi, int64_t B, int64_t T, int32_t N,
                       int32_t flags, int32_t strategy, int32_t chunk_len, void* workspace,
                       size_t workspace_bytes, void* hip_stream);
+
+/*
+ * Size of the leading workspace region that a forward and a backward with these
+ * arguments share (see TLPC_REUSE_SUMMARY), 0 when the backward cannot reuse the
+ * forward's summaries. A caller that wants the reuse gives the forward a
+ * workspace of at least the backward's size and passes the same one to the
+ * backward. No reference counterpart: the reference's backward
+ * (torchlpc/core.py:186-229) recomputes everything from A.
+ */
+size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,
+                              int32_t strategy, int32_t chunk_len);
 
 /*
  * First-order recurrence h[t] = w[t] * h[t-1] + x[t], h[-1] = init[b] (the order-1

@@ -76,6 +76,27 @@ def test_workspace_queries(lib):
     assert lib.tlpc_scan_workspace(TLPC_F32, 4, 1 << 20, 0, 0) > 0
 
 
+def test_summary_bytes(lib):
+    """The forward/backward shared region: the float64 transition matrices and
+    zero-state responses of chunks 0..P-2, for float32 order-16 time-varying
+    filters only."""
+    i64, i32 = ctypes.c_int64, ctypes.c_int32
+    f = lib.tlpc_lpc_summary_bytes
+    f.restype = ctypes.c_size_t
+    f.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32]
+    B, T, N, L = 256, 48000, 16, 384
+    P = -(-T // L)
+    summ = B * (P - 1) * (N * N + N) * 8
+    assert f(TLPC_F32, B, T, N, 0, 1, L) == -(-summ // 256) * 256
+    fwd = lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, N, 0, 1, L)
+    bwd = lib.tlpc_lpc_backward_workspace(TLPC_F32, B, T, N, NEED_DX | NEED_DA | NEED_DZI, 1, L)
+    assert fwd < bwd  # the backward adds its zero-state responses after the carries
+    assert f(TLPC_F64, B, T, N, 0, 1, L) == 0
+    assert f(TLPC_F32, B, T, 8, 0, 1, L) == 0
+    assert f(TLPC_F32, B, T, N, TI, 1, L) == 0
+    assert f(TLPC_F32, B, T, N, 0, 3, L) == 0  # serial strategy keeps no summaries
+
+
 def test_argument_validation_without_launch(lib):
     buf = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
     assert lib.tlpc_lpc_forward(9, buf, buf, buf, buf, 2, 8, 2, 0, 0, 0, None, 0, None) == TLPC_EDTYPE

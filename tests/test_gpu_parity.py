@@ -221,12 +221,61 @@ def test_headline_inputs_fp32(B, T):
     every 240 samples), on a subset of signals: y, dx, dA, dzi within 1e-5 of FP64."""
     x, A, zi, w = O.make_reflection_inputs(B, T, 16, 240, 1002)
     x, A, zi, w = (v.astype(np.float32) for v in (x, A, zi, w))
-    yd = ops().lpc(dev(x), dev(A), dev(zi))
-    dx, dA, dzi = ops().lpc_backward(dev(w), dev(A), dev(zi), yd, True, True, True)
+    yd, ws = ops().lpc_keep(dev(x), dev(A), dev(zi))
     y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
     rx, rA, rzi = O.lpc_backward(up(w, np.float32), up(A, np.float32), up(zi, np.float32), y64)
-    errs = [rel(host(yd), y64), rel(host(dx), rx), rel(host(dA), rA), rel(host(dzi), rzi)]
-    assert max(errs) <= 1e-5, errs
+    for summary in (None, ws):  # recomputed and reused transition matrices
+        dx, dA, dzi = ops().lpc_backward(dev(w), dev(A), dev(zi), yd, True, True, True, 0, 0, summary)
+        errs = [rel(host(yd), y64), rel(host(dx), rx), rel(host(dA), rA), rel(host(dzi), rzi)]
+        assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("T,chunk", [(16, 16), (17, 16), (383, 0), (385, 64), (2000, 0), (4097, 128)])
+def test_backward_reuses_forward_summaries(T, chunk):
+    """Order 16, float32: the backward that reuses the forward's transition
+    matrices (lpc_keep) gives bitwise the same gradients as the one that
+    recomputes them, and both meet the FP64 oracle."""
+    rng = np.random.default_rng(T * 3 + chunk)
+    x, A, zi = stable_inputs(rng, 3, T, 16, np.float32, kmax=0.8, hop=240)
+    g = stable_inputs(rng, 3, T, 1, np.float32)[0]
+    y, ws = ops().lpc_keep(dev(x), dev(A), dev(zi), 1, chunk)
+    # one chunk has no transition matrix to keep
+    assert (ws.numel() > 0) == (chunk == 0 or T > chunk), ws.numel()
+    summary = ws if ws.numel() > 0 else None
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    assert rel(host(y), y64) <= 1e-5
+    ref = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    for need in [(True, True, True), (True, False, False), (False, True, False), (False, False, True)]:
+        kept = ops().lpc_backward(dev(g), dev(A), dev(zi), y, *need, 1, chunk, summary)
+        fresh = ops().lpc_backward(dev(g), dev(A), dev(zi), y, *need, 1, chunk)
+        for flag, k, f, r in zip(need, kept, fresh, ref):
+            if flag:
+                assert torch.equal(k, f)
+                assert rel(host(k), r) <= 1e-5, (need, rel(host(k), r))
+    # the kept summaries survive a backward: a second one agrees bitwise
+    again = ops().lpc_backward(dev(g), dev(A), dev(zi), y, True, True, True, 1, chunk, summary)
+    first = ops().lpc_backward(dev(g), dev(A), dev(zi), y, True, True, True, 1, chunk)
+    assert all(torch.equal(a, b) for a, b in zip(again, first))
+
+
+def test_autograd_order16_uses_kept_summaries():
+    """LPC.apply on float32 order-16 inputs that require grad keeps the forward's
+    workspace for the backward; the gradients match the FP64 oracle."""
+    import torchlpc_amd
+
+    rng = np.random.default_rng(77)
+    x, A, zi = stable_inputs(rng, 2, 3000, 16, np.float32, kmax=0.8, hop=240)
+    g = stable_inputs(rng, 2, 3000, 1, np.float32)[0]
+    X, AA, Z = (dev(v).requires_grad_() for v in (x, A, zi))
+    y = torchlpc_amd.LPC.apply(X, AA, Z)
+    assert y.grad_fn is not None
+    y.backward(dev(g))
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    rx, rA, rzi = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    assert rel(host(y), y64) <= 1e-5
+    assert rel(host(X.grad), rx) <= 1e-5
+    assert rel(host(AA.grad), rA) <= 1e-5
+    assert rel(host(Z.grad), rzi) <= 1e-5
 
 
 @pytest.mark.parametrize("T,chunk", [(1, 0), (15, 0), (17, 16), (1000, 16), (3583, 64), (3585, 64), (48000, 0), (20000, 32)])

@@ -114,3 +114,24 @@ def test_autotune_table_roundtrip(tmp_path, monkeypatch):
     assert autotune.lookup("lpc_forward", x, A) == (1, 256)
     assert (tmp_path / "tune.json").exists()
     autotune.clear()
+
+
+def test_meta_order16_keeps_forward_summaries():
+    """A float32 order-16 forward that needs grad returns its workspace (fake kernel
+    sized by the C ABI queries) and the backward accepts it."""
+    from torchlpc_amd import _ext
+
+    x = meta(2, 5000).requires_grad_()
+    A = meta(2, 5000, 16).requires_grad_()
+    zi = meta(2, 16)
+    y, ws = torch.ops.torchlpc_amd.lpc_keep(x, A, zi)
+    assert y.shape == (2, 5000) and ws.dtype == torch.uint8
+    assert ws.numel() == _ext.keep_workspace_bytes(x, A, 0, 0) > 0
+    dx, dA, dzi = torch.ops.torchlpc_amd.lpc_backward(y, A, zi, y, True, True, False, 0, 0, ws)
+    assert dx.shape == x.shape and dA.shape == A.shape and dzi.numel() == 0
+    y = LPC.apply(x, A, zi)
+    gx, gA = torch.autograd.grad(y.sum(), (x, A))
+    assert gx.shape == x.shape and gA.shape == A.shape
+    # nothing to keep for float64 or a time-invariant a[B, N]
+    assert _ext.keep_workspace_bytes(meta(2, 5000, dtype=torch.float64), meta(2, 5000, 16, dtype=torch.float64), 0, 0) == 0
+    assert _ext.keep_workspace_bytes(x, meta(2, 16), 0, 0) == 0

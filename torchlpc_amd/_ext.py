@@ -14,6 +14,8 @@ PKG = Path(__file__).resolve().parent
 OPS_LIB = PKG / "_C.so"
 HIP_LIB = PKG / "libtorchlpc_amd_hip.so"
 
+_OPS = ("lpc", "lpc_keep", "lpc_backward", "scan", "scan_backward")
+
 EXTENSION_LOADED = False
 LOAD_ERROR: str | None = None
 
@@ -30,8 +32,46 @@ def _load() -> None:
     except (OSError, RuntimeError) as e:  # pragma: no cover - environment specific
         LOAD_ERROR = f"failed to load {OPS_LIB}: {e}"
         return
+    missing = [op for op in _OPS if not hasattr(torch.ops.torchlpc_amd, op)]
+    if missing:
+        LOAD_ERROR = (
+            f"{OPS_LIB} predates this package (missing ops {missing}); rebuild it with "
+            "`python -m torchlpc_amd._build`"
+        )
+        return
     EXTENSION_LOADED = True
     _register_fake()
+
+
+_DTYPE_CODE = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}
+
+
+def keep_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int, chunk_len: int) -> int:
+    """Bytes of the workspace ``lpc_keep`` returns for these inputs (0: nothing kept).
+
+    Host-side size queries of the C ABI (``tlpc_lpc_summary_bytes`` and the
+    forward/backward workspace queries); they launch nothing.
+    """
+    import ctypes
+
+    lib = ctypes.CDLL(str(HIP_LIB))
+    i64, i32 = ctypes.c_int64, ctypes.c_int32
+    for name in ("tlpc_lpc_summary_bytes", "tlpc_lpc_forward_workspace", "tlpc_lpc_backward_workspace"):
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_size_t
+        fn.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32]
+    dt = _DTYPE_CODE.get(x.dtype)
+    if dt is None:
+        return 0
+    B, T = x.shape
+    N = A.shape[-1]
+    ti = A.dim() == 2 or (A.shape[1] > 1 and A.stride(1) == 0)
+    flags = 8 if ti else 0
+    if lib.tlpc_lpc_summary_bytes(dt, B, T, N, flags, strategy, chunk_len) == 0:
+        return 0
+    fwd = lib.tlpc_lpc_forward_workspace(dt, B, T, N, flags, strategy, chunk_len)
+    bwd = lib.tlpc_lpc_backward_workspace(dt, B, T, N, flags | 7, strategy, chunk_len)
+    return max(fwd, bwd)
 
 
 def ops():
@@ -58,8 +98,13 @@ def _register_fake() -> None:
     def _lpc_fake(x, A, zi, strategy=0, chunk_len=0):
         return torch.empty_like(x)
 
+    @torch.library.register_fake(lib + "lpc_keep")
+    def _lpc_keep_fake(x, A, zi, strategy=0, chunk_len=0):
+        nbytes = keep_workspace_bytes(x, A, strategy, chunk_len)
+        return torch.empty_like(x), x.new_empty((nbytes,), dtype=torch.uint8)
+
     @torch.library.register_fake(lib + "lpc_backward")
-    def _lpc_bwd_fake(g, A, zi, y, need_x, need_a, need_zi, strategy=0, chunk_len=0):
+    def _lpc_bwd_fake(g, A, zi, y, need_x, need_a, need_zi, strategy=0, chunk_len=0, summary=None):
         B, T = g.shape
         N = A.shape[-1]
         dx = torch.empty_like(g) if need_x else g.new_empty((0,))

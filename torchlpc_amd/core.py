@@ -7,7 +7,10 @@ custom ``vmap``. The arithmetic runs in the HIP kernels of
 
 * forward: ``torch.ops.torchlpc_amd.lpc`` (chunked scan or per-signal kernel);
 * backward: ``torch.ops.torchlpc_amd.lpc_backward``, the fused adjoint that gives
-  dx, dA and dzi in one reverse-time pass. When the backward itself is being
+  dx, dA and dzi in one reverse-time pass. A forward whose inputs require grad
+  runs ``lpc_keep`` instead, which also returns its workspace: the chunk
+  transition matrices in it depend on A only, and the backward reuses them (the
+  adjoint's are their transposes). When the backward itself is being
   differentiated (``create_graph=True``), the gradients are instead expressed
   through ``LPC.apply`` (one more forward filter on the reversed, shifted
   coefficients, as the reference does at ``core.py:194-227``) so that they stay
@@ -19,6 +22,7 @@ read without being materialised.
 """
 from __future__ import annotations
 
+import threading
 from typing import Any, List, Optional, Tuple
 
 import torch
@@ -73,12 +77,28 @@ def composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi):
     return dx, dA, dzi
 
 
+# forward -> setup_context hand-over of the kept workspace (same thread, back to back)
+_kept = threading.local()
+
+
+def _keep_summaries(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> bool:
+    if torch.compiler.is_compiling():
+        return False
+    return x.requires_grad or A.requires_grad or zi.requires_grad
+
+
 class LPC(Function):
     """y[t] = x[t] - sum_{i=1..N} A[t,i] y[t-i], with y[-k] = zi[k-1]."""
 
     @staticmethod
     def forward(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> torch.Tensor:
         strategy, chunk = autotune.lookup("lpc_forward", x, A)
+        if _keep_summaries(x, A, zi):
+            # the backward runs with the forward's strategy and chunk length, so
+            # that the kept summaries match its chunking
+            y, ws = ops().lpc_keep(x, A, zi, strategy, chunk)
+            _kept.item = (y, ws, strategy, chunk) if ws.numel() > 0 else None
+            return y
         return ops().lpc(x, A, zi, strategy, chunk)
 
     @staticmethod
@@ -86,6 +106,11 @@ class LPC(Function):
         _, A, zi = inputs
         ctx.save_for_backward(A, zi, output)
         ctx.save_for_forward(A, zi, output)
+        kept = getattr(_kept, "item", None)
+        _kept.item = None
+        ctx.kept = None
+        if kept is not None and kept[0].data_ptr() == output.data_ptr():
+            ctx.kept = kept[1:]
 
     @staticmethod
     def backward(
@@ -95,10 +120,17 @@ class LPC(Function):
         need_x, need_A, need_zi = ctx.needs_input_grad[:3]
         if torch.is_grad_enabled():
             return composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi)
-        strategy, chunk = autotune.lookup("lpc_backward", grad_y, A)
-        dx, dA, dzi = ops().lpc_backward(
-            grad_y, A, zi, y, need_x, need_A, need_zi, strategy, chunk
-        )
+        kept = getattr(ctx, "kept", None)
+        if kept is not None:
+            ws, strategy, chunk = kept
+            dx, dA, dzi = ops().lpc_backward(
+                grad_y, A, zi, y, need_x, need_A, need_zi, strategy, chunk, ws
+            )
+        else:
+            strategy, chunk = autotune.lookup("lpc_backward", grad_y, A)
+            dx, dA, dzi = ops().lpc_backward(
+                grad_y, A, zi, y, need_x, need_A, need_zi, strategy, chunk
+            )
         return (dx if need_x else None, dA if need_A else None, dzi if need_zi else None)
 
     @staticmethod

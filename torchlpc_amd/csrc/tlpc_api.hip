@@ -78,8 +78,17 @@ struct Plan {
   int strategy = 0;
   int L = 0;
   int64_t P = 0;
-  size_t summ_off = 0, carry_off = 0, dapart_off = 0, total = 0;
+  bool c16 = false;  // float32 order-16 time-varying kernels
+  size_t summ_bytes = 0;
+  size_t summ_off = 0, carry_off = 0, wsum_off = 0, dapart_off = 0, total = 0;
 };
+
+// The order-16 float32 kernels serve time-varying filters with chunks that are
+// whole 16-step blocks and 32-bit sample indices inside a signal.
+inline bool chunk16_ok(int dtype, int32_t n, int32_t flags, int neg, int L, int64_t T) {
+  return dtype == TLPC_F32 && n == 16 && !(flags & TLPC_TIME_INVARIANT) && !neg && L % 16 == 0 &&
+         T * 16 < (int64_t(1) << 31);
+}
 
 Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
                int32_t chunk_len, bool backward, int neg = 0) {
@@ -107,9 +116,15 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
       (backward && (flags & TLPC_TIME_INVARIANT) && (flags & TLPC_NEED_DA) && !(flags & TLPC_DA_FULL))
           ? (size_t)B * p.P * p.np * es
           : 0;
+  p.c16 = p.strategy == TLPC_STRATEGY_CHUNKED && chunk16_ok(dtype, n, flags, neg, p.L, T);
+  // the order-16 adjoint keeps its zero-state responses apart from the summaries,
+  // which hold the forward's transition matrices
+  const size_t wsum = (backward && p.c16) ? (size_t)B * (p.P > 1 ? p.P - 1 : 0) * p.np * ws_es : 0;
   p.summ_off = 0;
-  p.carry_off = align_up(summ);
-  p.dapart_off = p.carry_off + align_up(carry);
+  p.summ_bytes = align_up(summ);
+  p.carry_off = p.summ_bytes;
+  p.wsum_off = p.carry_off + align_up(carry);
+  p.dapart_off = p.wsum_off + align_up(wsum);
   p.total = p.dapart_off + align_up(dapart);
   return p;
 }
@@ -194,6 +209,7 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   using W = typename Wide<S>::type;
   a.summ = reinterpret_cast<W*>(w + p.summ_off);
   a.carry = reinterpret_cast<W*>(w + p.carry_off);
+  a.wsum = reinterpret_cast<W*>(w + p.wsum_off);
   if (need_da && da_reduced) {
     a.dapart = reinterpret_cast<S*>(w + p.dapart_off);
     a.da_ti = static_cast<S*>(dA);
@@ -202,8 +218,14 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   }
   if constexpr (sizeof(S) == 4) {
     // float32, order 16, time-varying: DPP summaries and quad reruns
-    if (n == 16 && !ti && !neg && p.L % 16 == 0) {
-      int rc = backward ? launch_chunk16_adj_summary(a, st) : launch_chunk16_fwd_summary(a, st);
+    if (p.c16) {
+      int rc = TLPC_OK;
+      if (!backward) {
+        rc = launch_chunk16_fwd_summary(a, true, st);
+      } else {
+        if (!(flags & TLPC_REUSE_SUMMARY)) rc = launch_chunk16_fwd_summary(a, false, st);
+        if (rc == TLPC_OK) rc = launch_chunk16_adj_summary(a, st);
+      }
       if (rc == TLPC_OK) rc = launch_chunk16_carry(a, !backward, st);
       if (rc == TLPC_OK) rc = backward ? launch_chunk16_adj_rerun(a, st) : launch_chunk16_fwd_rerun(a, st);
       return rc;
@@ -285,6 +307,15 @@ TLPC_EXPORT int tlpc_lpc_backward(int dtype, const void* g, const void* A, const
                                   void* hip_stream) {
   return run_dtype(dtype, g, A, zi, const_cast<void*>(y), dx, dA, dzi, B, T, N, flags, strategy,
                    chunk_len, workspace, workspace_bytes, hip_stream, true);
+}
+
+TLPC_EXPORT size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32_t N,
+                                          int32_t flags, int32_t strategy, int32_t chunk_len) {
+  if (check_common(dtype, B, T, N) != TLPC_OK) return 0;
+  const Plan f = make_plan(dtype, B, T, N, flags & TLPC_TIME_INVARIANT, strategy, chunk_len, false);
+  const Plan b = make_plan(dtype, B, T, N, flags, strategy, chunk_len, true);
+  if (!f.c16 || !b.c16 || f.L != b.L || f.summ_bytes != b.summ_bytes) return 0;
+  return f.summ_bytes;
 }
 
 TLPC_EXPORT size_t tlpc_scan_workspace(int dtype, int64_t B, int64_t T, int32_t flags,

@@ -1,11 +1,16 @@
 // Chunked scan for order 16, float32 data (the B=256, T=48000, N=16 shape and
 // any other float32 order-16 batch), built from the DPP blocks of tlpc_dpp16.h:
 //
-//   summary  16 chunks per workgroup: waves 0-3 are 16 matrix rows (one chunk
-//            per 16-lane row, float64 basis responses -> Phi), wave 4 is 16 quads
-//            (float64 zero-state response -> z). Forward: chunks 0..P-2; adjoint
-//            (transposed recurrence, reverse time): chunks 1..P-1.
-//   carry    the generic per-signal chain (tlpc_chunked_impl.h), float64.
+//   summary  forward: 16 chunks per workgroup, waves 0-3 are 16 matrix rows (one
+//            chunk per 16-lane row, float64 basis responses -> Phi), wave 4 is 16
+//            quads (float64 zero-state response -> z); chunks 0..P-2.
+//            adjoint: quads only (zero-state response of the transposed
+//            recurrence -> w), chunks 1..P-1. The adjoint transition matrix of a
+//            chunk is Phi^T: the transposed recurrence is the adjoint of the
+//            forward one, so its homogeneous chunk map is the transpose. An
+//            adjoint that is handed the forward's summaries skips Phi entirely;
+//            otherwise it computes Phi with the forward's matrix rows.
+//   carry    one wave per signal, float64.
 //   rerun    one quad per chunk from its float64 carry, float64 arithmetic on the
 //            float32 data (the rerun produces the outputs, so it keeps their
 //            error at the final float32 rounding);
@@ -13,8 +18,8 @@
 //            the quad's four lanes (one float4 each, a coalesced row store) and
 //            dzi for chunk 0.
 //
-// Summaries and carries use the generic layout: summ[chunk][r*16 + k] = Phi[r][k],
-// summ[chunk][256 + r] = z[r]; carry[b][c][r].
+// Layouts: summ[b][c][r*16 + k] = Phi_c[r][k], summ[b][c][256 + r] = z_c[r]
+// (c = 0..P-2); wsum[b][c-1][r] = w_c[r] (c = 1..P-1); carry[b][c][r].
 #include "tlpc_dpp16.h"
 #include "tlpc_kernels.h"
 
@@ -27,185 +32,262 @@ constexpr int SUM_THREADS = 320;  // 4 matrix waves + 1 quad wave = 16 chunks
 constexpr int RERUN_THREADS = 256;
 
 // ---------------------------------------------------------------------------
-// Matrix rows.
-template <int J>
-__device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], const float* ap,
-                                              int jb, int L) {
-  if constexpr (J < NP) {
-    const int jn = jb + J + 8;
-    cf[(J + 8) & 15] = (jn < L) ? ap[(int64_t)jn * NP] : 0.f;
-    r[J] = fwd_row_step<J>((double)cf[J], r);
-    fwd_row_block<J + 1>(r, cf, ap, jb, L);
-  }
+// Loads. Every load in the step loops is unconditional, at an index clamped into
+// the signal; values that must read as zero are selected afterwards. A load under
+// a branch leaves the compiler a phi to merge, and it waits for the load at the
+// merge, which serialises the prefetch. Indices inside one signal are 32-bit: the
+// dispatch keeps T * 16 below 2^31.
+__device__ __forceinline__ float4 ld_row(const float* Ab, int t, int ql) {
+  return reinterpret_cast<const float4*>(Ab + t * NP)[ql];
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// y_u of one signal, with the initial conditions y_{-1-k} = zi[k] (k < 16) and
+// zero elsewhere before 0: the address is always valid (indices past T clamp to
+// T-1, a value the callers never use) and yok says whether the value counts. The
+// two are applied at different steps so that the load is never made conditional.
+__device__ __forceinline__ const float* yaddr(const float* yb, const float* zib, int u, int T) {
+  return u >= 0 ? yb + min(u, T - 1) : (zib ? zib + min(-u - 1, NP - 1) : yb);
+}
+__device__ __forceinline__ bool yok(const float* zib, int u) {
+  return u >= 0 || (zib != nullptr && -u - 1 < NP);
 }
 
-// Descending steps jb+15 .. jb; step jb+JJ reads cf[JJ] and loads step jb+JJ-8.
-template <int JJ>
-__device__ __forceinline__ void adj_row_block(double (&p)[NP], float (&cf)[NP], const float* ap,
-                                              int jb, int64_t t0, int64_t T) {
-  if constexpr (JJ >= 0) {
-    const int jn = jb + JJ - 8;
-    const float a = cf[JJ];
-    cf[(JJ + 8) & 15] = (jn >= 0 && t0 + jn < T) ? ap[(int64_t)jn * NP] : 0.f;
-    adj_row_step<JJ>((double)a, p);
-    adj_row_block<JJ - 1>(p, cf, ap, jb, t0, T);
+template <int K>
+__device__ __forceinline__ void put(float4& o, int ql, float v) {
+  // component K & 3 of lane K >> 2 of the quad
+  const bool mine = ql == (K >> 2);
+  if constexpr ((K & 3) == 0) o.x = mine ? v : o.x;
+  if constexpr ((K & 3) == 1) o.y = mine ? v : o.y;
+  if constexpr ((K & 3) == 2) o.z = mine ? v : o.z;
+  if constexpr ((K & 3) == 3) o.w = mine ? v : o.w;
+}
+
+// Stores the 16 samples [tb, tb+16) that a quad gathered (lane ql holds
+// tb+4ql .. tb+4ql+3): one float4 per lane when the block is whole and aligned.
+__device__ __forceinline__ void store16(float* vb, int tb, int T, int ql, float4 o) {
+  float* p = vb + tb + 4 * ql;
+  if (tb + 16 <= T && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    *reinterpret_cast<float4*>(p) = o;
+  } else {
+    const int u = tb + 4 * ql;
+    if (u < T) p[0] = o.x;
+    if (u + 1 < T) p[1] = o.y;
+    if (u + 2 < T) p[2] = o.z;
+    if (u + 3 < T) p[3] = o.w;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Quad runs (prefetching PF rows ahead).
+// Matrix rows. Lane kk of a 16-lane row reads A[t, kk+1] (ap = &A[t0, kk]);
+// cf is a 16-slot ring, loads run RD = 15 steps ahead (into the slot the
+// previous step freed).
+constexpr int RD = 15;
+template <int J>
+__device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], const float* ap,
+                                              int jb, int jlim) {
+  if constexpr (J < NP) {
+    cf[(J + RD) & 15] = ap[min(jb + J + RD, jlim) * NP];
+    r[J] = fwd_row_step<J>((double)cf[J], r);
+    fwd_row_block<J + 1>(r, cf, ap, jb, jlim);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward quad run over [t0, t0+len): 16-step blocks, a PF-deep ring of rows and
+// inputs. Steps past the end of the signal run on clamped data and store nothing.
+struct FwdIo {
+  const float* Ab;  // A of the signal, [T][16]
+  const float* xb;
+  float* yb;
+  int T;
+};
+
 template <int JJ, bool STORE, typename V>
-__device__ __forceinline__ void fwd_quad_block(V (&R)[4], float4 (&cq)[PF], float (&xq)[PF],
-                                               const float* Ab, const float* xb, float* yb,
-                                               int64_t t0, int jb, int len, int ql) {
-  if constexpr (JJ < PF) {
-    const int step = jb + JJ;
-    if (step < len) {
-      const int64_t t = t0 + step;
-      const float4 c = cq[JJ];
-      const float xv = xq[JJ];
-      if (step + PF < len) {
-        cq[JJ] = reinterpret_cast<const float4*>(Ab + (t + PF) * NP)[ql];
-        xq[JJ] = xb[t + PF];
-      }
-      const V y = fwd_quad_step<JJ & 3>(R, c, xv, ql);
-      if (STORE && ((t & 3) == ql)) yb[t] = (float)y;
-    }
-    fwd_quad_block<JJ + 1, STORE>(R, cq, xq, Ab, xb, yb, t0, jb, len, ql);
+__device__ __forceinline__ void fwd_q16(V (&R)[4], float4 (&cq)[PF], float (&xq)[PF], float4& out,
+                                        const FwdIo& io, int tb, int ql) {
+  if constexpr (JJ < 16) {
+    const float4 c = cq[JJ % PF];
+    const float xv = xq[JJ % PF];
+    const int tn = min(tb + JJ + PF, io.T - 1);
+    cq[JJ % PF] = ld_row(io.Ab, tn, ql);
+    xq[JJ % PF] = io.xb[tn];
+    const V y = fwd_quad_step<JJ & 3>(R, c, xv, ql);
+    if constexpr (STORE) put<JJ>(out, ql, (float)y);
+    fwd_q16<JJ + 1, STORE>(R, cq, xq, out, io, tb, ql);
   }
 }
 
 template <bool STORE, typename V>
-__device__ __forceinline__ void fwd_quad_run(V (&R)[4], const float* Ab, const float* xb,
-                                             float* yb, int64_t t0, int len, int ql) {
+__device__ __forceinline__ void fwd_quad_run(V (&R)[4], const FwdIo& io, int t0, int len, int ql) {
   float4 cq[PF];
   float xq[PF];
 #pragma unroll
   for (int j = 0; j < PF; ++j) {
-    if (j < len) {
-      cq[j] = reinterpret_cast<const float4*>(Ab + (t0 + j) * NP)[ql];
-      xq[j] = xb[t0 + j];
+    const int tj = min(t0 + j, io.T - 1);
+    cq[j] = ld_row(io.Ab, tj, ql);
+    xq[j] = io.xb[tj];
+  }
+  for (int jb = 0; jb < len; jb += 16) {
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    fwd_q16<0, STORE>(R, cq, xq, out, io, t0 + jb, ql);
+    if constexpr (STORE) store16(io.yb, t0 + jb, io.T, ql, out);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint quad run, descending over the 16-step blocks of [t0, t0+L). Steps at
+// or past T are no-ops of a zero state (zero coefficients and gradient). With
+// OUT it also emits dx (gathered per block), the dA rows (when DA) and keeps the
+// window Y of y_{t-1-4ql-m} for them.
+struct AdjIo {
+  const float* Ab;
+  const float* gb;
+  const float* yb;
+  const float* zib;
+  float* dxb;
+  float* dAb;
+  int T;
+};
+
+template <int JJ, bool OUT, bool DA>
+__device__ __forceinline__ void adj_q16(double (&P)[4], double& now, float (&Y)[4],
+                                        float4 (&cq)[PF], float (&gq)[PF], float (&yq)[PF],
+                                        float4& dxo, const AdjIo& io, int tb, int ql) {
+  if constexpr (JJ < 16) {
+    constexpr int J = (3 - JJ) & 3;  // slot of t = tb + 15 - JJ (tb is a multiple of 16)
+    const int t = tb + 15 - JJ;
+    const float4 cv = cq[JJ % PF];
+    const float gv = gq[JJ % PF];
+    const float yraw = yq[JJ % PF];
+    {
+      const int tc = clampi(t - PF, 0, io.T - 1);
+      cq[JJ % PF] = ld_row(io.Ab, tc, ql);
+      gq[JJ % PF] = io.gb[tc];
+      if constexpr (DA) yq[JJ % PF] = *yaddr(io.yb, io.zib, t - PF - 17, io.T);
+    }
+    const bool live = t < io.T;
+    const double lam = adj_quad_step<J>(P, now, cv, gv, ql, live);
+    if constexpr (OUT) {
+      const float lf = (float)lam;
+      put<15 - JJ>(dxo, ql, lf);
+      if constexpr (DA) {
+        if (live) {
+          float4 d;
+          d.x = -lf * Y[(J - 1) & 3];
+          d.y = -lf * Y[(J - 2) & 3];
+          d.z = -lf * Y[(J - 3) & 3];
+          d.w = -lf * Y[(J - 4) & 3];
+          reinterpret_cast<float4*>(io.dAb + t * NP)[ql] = d;
+        }
+        constexpr int S = (J - 1) & 3;
+        const float down = qperm<QP_DOWN>(Y[S]);
+        const float ynew = yok(io.zib, t - 17) ? yraw : 0.f;
+        Y[S] = (ql == 3) ? ynew : down;  // lane 3 takes y_{t-17}
+      }
+    }
+    adj_q16<JJ + 1, OUT, DA>(P, now, Y, cq, gq, yq, dxo, io, tb, ql);
+  }
+}
+
+template <bool OUT, bool DA>
+__device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const AdjIo& io, int t0,
+                                             int L, int ql) {
+  const int tstart = t0 + L - 1;
+  // y window: slot u mod 4 holds y_u, u in [t-4-4ql, t-1-4ql] for t = tstart
+  // (values past T never meet a nonzero lambda)
+  float Y[4] = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (DA) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int u = tstart - 1 - 4 * ql - m;
+      const float v = *yaddr(io.yb, io.zib, u, io.T);
+      Y[(2 - m) & 3] = yok(io.zib, u) ? v : 0.f;
     }
   }
-  for (int jb = 0; jb < len; jb += PF) fwd_quad_block<0, STORE>(R, cq, xq, Ab, xb, yb, t0, jb, len, ql);
+  float4 cq[PF];
+  float gq[PF], yq[PF];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int tc = clampi(tstart - j, 0, io.T - 1);
+    cq[j] = ld_row(io.Ab, tc, ql);
+    gq[j] = io.gb[tc];
+    yq[j] = DA ? *yaddr(io.yb, io.zib, tstart - j - 17, io.T) : 0.f;
+  }
+  for (int tb = t0 + L - 16; tb >= t0; tb -= 16) {
+    float4 dxo = make_float4(0.f, 0.f, 0.f, 0.f);
+    adj_q16<0, OUT, DA>(P, now, Y, cq, gq, yq, dxo, io, tb, ql);
+    if constexpr (OUT) {
+      if (io.dxb) store16(io.dxb, tb, io.T, ql, dxo);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Forward summaries: chunks c = 0..P-2 (full length L).
+template <bool WITH_Z>
 __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t cps = a.P - 1;
   const int64_t nch = a.B * cps;
   const int L = a.L;
+  const int T = (int)a.T;
   if (wave < 4) {
     const int64_t cid = (int64_t)blockIdx.x * 16 + wave * 4 + (lane >> 4);
     if (cid >= nch) return;
     const int kk = lane & 15;
     const int64_t b = cid / cps;
-    const int64_t c = cid - b * cps;
-    const float* ap = a.A + (b * a.T + c * L) * NP + kk;
+    const int c = (int)(cid - b * cps);
+    const int t0 = c * L;
+    const float* ap = a.A + (b * a.T + t0) * NP + kk;
+    const int jlim = T - 1 - t0;
     double r[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) r[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // y_{-1-kk} = 1
     float cf[NP];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) cf[j] = ap[(int64_t)j * NP];
-    for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap, jb, L);
+    for (int j = 0; j < RD; ++j) cf[j] = ap[j * NP];
+    for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap, jb, jlim);
     double* out = a.summ + cid * (NP * NP + NP);
 #pragma unroll
     for (int rr = 0; rr < NP; ++rr) out[rr * NP + kk] = r[NP - 1 - rr];
-  } else {
+  } else if (WITH_Z) {
     const int64_t cid = (int64_t)blockIdx.x * 16 + (lane >> 2);
     if (cid >= nch) return;
     const int ql = lane & 3;
     const int64_t b = cid / cps;
-    const int64_t c = cid - b * cps;
+    const int c = (int)(cid - b * cps);
     double R[4] = {0.0, 0.0, 0.0, 0.0};
-    fwd_quad_run<false>(R, a.A + b * a.T * NP, a.x + b * a.T, nullptr, c * L, L, ql);
+    const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, T};
+    fwd_quad_run<false>(R, io, c * L, L, ql);
     double* out = a.summ + cid * (NP * NP + NP) + NP * NP;
 #pragma unroll
     for (int m = 0; m < 4; ++m) out[4 * ql + m] = R[3 - m];  // z[4ql+m] = y_{L-1-4ql-m}
   }
 }
 
-// Adjoint summaries: chunks c = 1..P-1 (the last may be partial; its steps past
-// the end are no-ops of the zero state, and its Psi is never used).
-__global__ __launch_bounds__(SUM_THREADS) void sum_adj(ChunkArgs<float> a) {
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+// Adjoint zero-state responses: chunks c = 1..P-1, one quad each (the last may
+// be partial; its steps past the end are no-ops of the zero state).
+constexpr int WSUM_THREADS = 256;
+__global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
   const int64_t cps = a.P - 1;
-  const int64_t nch = a.B * cps;
-  const int L = a.L;
-  const int64_t T = a.T;
-  if (wave < 4) {
-    const int64_t cid = (int64_t)blockIdx.x * 16 + wave * 4 + (lane >> 4);
-    if (cid >= nch) return;
-    const int kk = lane & 15;
-    const int64_t b = cid / cps;
-    const int64_t c = cid - b * cps + 1;
-    const int64_t t0 = c * L;
-    const float* ap = a.A + (b * T + t0) * NP + kk;
-    double p[NP];
+  const int64_t cid = ((int64_t)blockIdx.x * WSUM_THREADS + threadIdx.x) >> 2;
+  if (cid >= a.B * cps) return;
+  const int ql = threadIdx.x & 3;
+  const int64_t b = cid / cps;
+  const int c = (int)(cid - b * cps) + 1;
+  double P[4] = {0.0, 0.0, 0.0, 0.0};
+  double now = 0.0;
+  const AdjIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, nullptr, nullptr, nullptr, (int)a.T};
+  adj_quad_run<false, false>(P, now, io, c * a.L, a.L, ql);
+  // w[0] = now (lane 0); w[4ql+1+m] = P[(2-m) & 3]
+  double* out = a.wsum + cid * NP;
+  if (ql == 0) out[0] = now;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) p[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // pending(t1-1-kk) = 1
-    float cf[NP];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int s = L - 1 - j;
-      cf[15 - j] = (t0 + s < T) ? ap[(int64_t)s * NP] : 0.f;
-    }
-    for (int jb = L - NP; jb >= 0; jb -= NP) adj_row_block<15>(p, cf, ap, jb, t0, T);
-    double* out = a.summ + cid * (NP * NP + NP);
-#pragma unroll
-    for (int rr = 0; rr < NP; ++rr) out[rr * NP + kk] = p[NP - 1 - rr];
-  } else {
-    const int64_t cid = (int64_t)blockIdx.x * 16 + (lane >> 2);
-    if (cid >= nch) return;
-    const int ql = lane & 3;
-    const int64_t b = cid / cps;
-    const int64_t c = cid - b * cps + 1;
-    const int64_t t0 = c * L;
-    const float* Ab = a.A + b * T * NP;
-    const float* gb = a.x + b * T;
-    double P[4] = {0.0, 0.0, 0.0, 0.0};
-    double now = 0.0;
-    // descending steps L-1 .. 0, PF rows in flight
-    float4 cq[PF];
-    float gq[PF];
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int j = 0; j < PF; ++j) {
-      const int64_t t = t0 + L - 1 - j;
-      cq[j] = (t < T) ? reinterpret_cast<const float4*>(Ab + t * NP)[ql] : zero4;
-      gq[j] = (t < T) ? gb[t] : 0.f;
-    }
-    for (int jb = 0; jb < L; jb += PF) {
-      // step index from the top: s = jb + jj, t = t0 + L - 1 - s, slot (3 - jj) & 3
-#define TLPC_ADJ_Q(JJ)                                                                 \
-  {                                                                                    \
-    const int s = jb + JJ;                                                             \
-    const float4 cv = cq[JJ];                                                          \
-    const float gv = gq[JJ];                                                           \
-    const int64_t tn = t0 + L - 1 - (s + PF);                                          \
-    if (s + PF < L) {                                                                  \
-      cq[JJ] = (tn < T) ? reinterpret_cast<const float4*>(Ab + tn * NP)[ql] : zero4;   \
-      gq[JJ] = (tn < T) ? gb[tn] : 0.f;                                                \
-    }                                                                                  \
-    adj_quad_step<(3 - JJ) & 3>(P, now, cv, gv, ql);                                   \
-  }
-      TLPC_ADJ_Q(0) TLPC_ADJ_Q(1) TLPC_ADJ_Q(2) TLPC_ADJ_Q(3)
-      TLPC_ADJ_Q(4) TLPC_ADJ_Q(5) TLPC_ADJ_Q(6) TLPC_ADJ_Q(7)
-#undef TLPC_ADJ_Q
-    }
-    // w[0] = now (lane 0); w[4ql+1+m] = P[(2-m) & 3]
-    double* out = a.summ + cid * (NP * NP + NP) + NP * NP;
-    if (ql == 0) out[0] = now;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int rr = 4 * ql + 1 + m;
-      if (rr < NP) out[rr] = P[(2 - m) & 3];
-    }
+  for (int m = 0; m < 4; ++m) {
+    const int rr = 4 * ql + 1 + m;
+    if (rr < NP) out[rr] = P[(2 - m) & 3];
   }
 }
 
@@ -216,34 +298,29 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
   if (cid >= a.B * a.P) return;
   const int ql = threadIdx.x & 3;
   const int64_t b = cid / a.P;
-  const int64_t c = cid - b * a.P;
-  const int64_t t0 = c * a.L;
-  const int len = (int)((a.T - t0) < a.L ? (a.T - t0) : a.L);
+  const int c = (int)(cid - b * a.P);
+  const int T = (int)a.T;
+  const int t0 = c * a.L;
+  const int len = min(T - t0, a.L);
   const double* cr = a.carry + cid * NP;
   double R[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) R[3 - m] = cr[4 * ql + m];  // y_{t0-1-4ql-m}, slot 3-m
-  fwd_quad_run<true>(R, a.A + b * a.T * NP, a.x + b * a.T, a.y + b * a.T, t0, len, ql);
+  const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, a.y + b * a.T, T};
+  fwd_quad_run<true>(R, io, t0, len, ql);
 }
 
 // Adjoint rerun: one quad per chunk, descending from the chunk's last step (steps
 // past the end of the last chunk are no-ops of its zero carry).
+template <bool DA>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
   const int64_t cid = ((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2;
   if (cid >= a.B * a.P) return;
   const int ql = threadIdx.x & 3;
   const int64_t b = cid / a.P;
-  const int64_t c = cid - b * a.P;
-  const int64_t T = a.T;
+  const int c = (int)(cid - b * a.P);
+  const int T = (int)a.T;
   const int L = a.L;
-  const int64_t t0 = c * L;
-  const float* Ab = a.A + b * T * NP;
-  const float* gb = a.x + b * T;
-  const float* yb = a.y + b * T;
-  const float* zib = a.zi ? a.zi + b * NP : nullptr;
-  float* dxb = a.dx ? a.dx + b * T : nullptr;
-  float* dAb = a.dA ? a.dA + b * T * NP : nullptr;
-  const bool need_y = dAb != nullptr;
   // pending state from the carry: now = p_0, window slot (2-m)&3 = p_{4ql+1+m}
   const double* cr = a.carry + cid * NP;
   double P[4];
@@ -253,67 +330,14 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
     const int rr = 4 * ql + 1 + m;
     P[(2 - m) & 3] = rr < NP ? cr[rr] : 0.0;
   }
-  auto yval = [&](int64_t u) -> float {
-    if (u >= 0) return u < T ? yb[u] : 0.f;
-    return (zib && -u - 1 < NP) ? zib[-u - 1] : 0.f;
-  };
-  // y window: slot u mod 4 holds y_u, u in [t-4-4ql, t-1-4ql], t = t0 + L - 1
-  float Y[4] = {0.f, 0.f, 0.f, 0.f};
-  const int64_t tstart = t0 + L - 1;
-  if (need_y) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int64_t u = tstart - 1 - 4 * ql - m;
-      Y[(2 - m) & 3] = yval(u);  // (tstart - 1 - m) mod 4 = (2 - m) & 3
-    }
-  }
-  float4 cq[PF];
-  float gq[PF], yq[PF];
-  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-  for (int j = 0; j < PF; ++j) {
-    const int64_t t = tstart - j;
-    cq[j] = (t < T) ? reinterpret_cast<const float4*>(Ab + t * NP)[ql] : zero4;
-    gq[j] = (t < T) ? gb[t] : 0.f;
-    yq[j] = need_y ? yval(t - 17) : 0.f;  // enters lane 3's window after step t
-  }
-  for (int jb = 0; jb < L; jb += PF) {
-#define TLPC_ADJ_R(JJ)                                                                        \
-  {                                                                                           \
-    constexpr int J = (3 - JJ) & 3;                                                           \
-    const int s = jb + JJ;                                                                    \
-    const int64_t t = tstart - s;                                                             \
-    const float4 cv = cq[JJ];                                                                 \
-    const float gv = gq[JJ];                                                                  \
-    const float ynew = yq[JJ];                                                                \
-    const int64_t tn = t - PF;                                                                \
-    if (s + PF < L) {                                                                         \
-      cq[JJ] = (tn < T) ? reinterpret_cast<const float4*>(Ab + tn * NP)[ql] : zero4;          \
-      gq[JJ] = (tn < T) ? gb[tn] : 0.f;                                                       \
-      yq[JJ] = need_y ? yval(tn - 17) : 0.f;                                                  \
-    }                                                                                         \
-    const double lam = adj_quad_step<J>(P, now, cv, gv, ql);                                  \
-    if (t < T) {                                                                              \
-      if (dxb && (t & 3) == ql) dxb[t] = (float)lam;                                          \
-      if (need_y) {                                                                           \
-        float4 d;                                                                             \
-        d.x = (float)(-lam * Y[(J - 1) & 3]);                                                 \
-        d.y = (float)(-lam * Y[(J - 2) & 3]);                                                 \
-        d.z = (float)(-lam * Y[(J - 3) & 3]);                                                 \
-        d.w = (float)(-lam * Y[(J - 4) & 3]);                                                 \
-        reinterpret_cast<float4*>(dAb + t * NP)[ql] = d;                                      \
-      }                                                                                       \
-    }                                                                                         \
-    if (need_y) {                                                                             \
-      constexpr int S = (J - 1) & 3;                                                          \
-      const float down = qperm<QP_DOWN>(Y[S]);                                                \
-      Y[S] = (ql == 3) ? ynew : down;                                                         \
-    }                                                                                         \
-  }
-    TLPC_ADJ_R(0) TLPC_ADJ_R(1) TLPC_ADJ_R(2) TLPC_ADJ_R(3)
-    TLPC_ADJ_R(4) TLPC_ADJ_R(5) TLPC_ADJ_R(6) TLPC_ADJ_R(7)
-#undef TLPC_ADJ_R
-  }
+  const AdjIo io{a.A + b * a.T * NP,
+                 a.x + b * a.T,
+                 a.y + b * a.T,
+                 a.zi ? a.zi + b * NP : nullptr,
+                 a.dx ? a.dx + b * a.T : nullptr,
+                 a.dA ? a.dA + b * a.T * NP : nullptr,
+                 T};
+  adj_quad_run<true, DA>(P, now, io, c * L, L, ql);
   if (c == 0 && a.dzi) {
     // pending of positions -1-r: r = 0 in `now` (lane 0), r = 4ql+1+m in slot (2-m)&3
     float* dz = a.dzi + b * NP;
@@ -327,8 +351,12 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Carry chain, one wave per signal. Summaries come through LDS in batches of
-// CB links: the loads of batch n+1 are in flight while batch n is chained.
+// Carry chain, one wave per signal: s_{l+1} = M_l s_l + v_l over the P-1 links.
+// Forward: link l is chunk l, M = Phi_l, v = z_l. Adjoint: link l is chunk
+// P-1-l, M = Phi_{P-1-l}^T, v = w_{P-1-l}; the first link (the last chunk) meets
+// the zero state past the end of the signal, so it needs no matrix. Link data
+// comes through LDS in batches of CB links: the loads of batch n+1 are in flight
+// while batch n is chained.
 constexpr int CB = 8;
 constexpr int SUMM = NP * NP + NP;
 
@@ -340,10 +368,8 @@ __global__ __launch_bounds__(64) void carry16(ChunkArgs<float> a) {
   const int64_t P = a.P;
   const int64_t links = P - 1;
   const double* summ = a.summ + b * links * SUMM;
+  const double* wsum = FWD ? nullptr : a.wsum + b * links * NP;
   double* carry = a.carry + b * P * NP;
-  // link l maps the carry of step l to step l+1; forward: chunk l, summary l;
-  // adjoint: chunk P-1-l, summary P-2-l
-  auto src = [&](int64_t l) -> const double* { return summ + (FWD ? l : links - 1 - l) * SUMM; };
   constexpr int PER = CB * SUMM / 64;  // doubles per lane per batch
   double stage[PER];
   auto fetch = [&](int64_t l0) {
@@ -351,7 +377,18 @@ __global__ __launch_bounds__(64) void carry16(ChunkArgs<float> a) {
     for (int q = 0; q < PER; ++q) {
       const int e = q * 64 + lane;
       const int li = e / SUMM, off = e - li * SUMM;
-      stage[q] = (l0 + li < links) ? src(l0 + li)[off] : 0.0;
+      const int64_t l = l0 + li;
+      double v = 0.0;
+      if (l < links) {
+        if (FWD) {
+          v = summ[l * SUMM + off];
+        } else if (off >= NP * NP) {
+          v = wsum[(links - 1 - l) * NP + off - NP * NP];
+        } else if (l > 0) {
+          v = summ[(links - l) * SUMM + off];
+        }
+      }
+      stage[q] = v;
     }
   };
   auto commit = [&](int bb) {
@@ -377,7 +414,7 @@ __global__ __launch_bounds__(64) void carry16(ChunkArgs<float> a) {
       if (lane < NP) carry[(FWD ? l : P - 1 - l) * NP + lane] = s;
       double m[NP];
 #pragma unroll
-      for (int k = 0; k < NP; ++k) m[k] = buf[bb][li][row * NP + k];
+      for (int k = 0; k < NP; ++k) m[k] = FWD ? buf[bb][li][row * NP + k] : buf[bb][li][k * NP + row];
       s = chain_link(s, m, buf[bb][li][NP * NP + row]);
     }
     if (l0 + CB < links) commit(bb ^ 1);
@@ -395,16 +432,20 @@ int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-int launch_chunk16_fwd_summary(ChunkArgs<float> a, hipStream_t st) {
+int launch_chunk16_fwd_summary(ChunkArgs<float> a, bool with_z, hipStream_t st) {
   const int64_t n = a.B * (a.P - 1);
-  if (n > 0)
-    hipLaunchKernelGGL(c16::sum_fwd, dim3((unsigned)((n + 15) / 16)), dim3(c16::SUM_THREADS), 0, st, a);
+  const dim3 grid((unsigned)((n + 15) / 16));
+  if (n > 0 && with_z)
+    hipLaunchKernelGGL(c16::sum_fwd<true>, grid, dim3(c16::SUM_THREADS), 0, st, a);
+  else if (n > 0)
+    hipLaunchKernelGGL(c16::sum_fwd<false>, grid, dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_adj_summary(ChunkArgs<float> a, hipStream_t st) {
-  const int64_t n = a.B * (a.P - 1);
-  if (n > 0)
-    hipLaunchKernelGGL(c16::sum_adj, dim3((unsigned)((n + 15) / 16)), dim3(c16::SUM_THREADS), 0, st, a);
+  const int64_t lanes = a.B * (a.P - 1) * 4;
+  if (lanes > 0)
+    hipLaunchKernelGGL(c16::sum_adj, dim3((unsigned)((lanes + c16::WSUM_THREADS - 1) / c16::WSUM_THREADS)),
+                       dim3(c16::WSUM_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st) {
@@ -415,8 +456,11 @@ int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st) {
 }
 int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * a.P * 4;
-  hipLaunchKernelGGL(c16::rerun_adj, dim3((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS)),
-                     dim3(c16::RERUN_THREADS), 0, st, a);
+  const dim3 grid((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS));
+  if (a.dA)
+    hipLaunchKernelGGL(c16::rerun_adj<true>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else
+    hipLaunchKernelGGL(c16::rerun_adj<false>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

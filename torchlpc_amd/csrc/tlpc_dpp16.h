@@ -17,7 +17,7 @@ namespace tlpc {
 namespace dpp16 {
 
 constexpr int NP = 16;
-constexpr int PF = 8;  // rows in flight per quad lane
+constexpr int PF = 16;  // rows in flight per quad lane (one 16-step block)
 
 // ---------------------------------------------------------------------------
 // Forward matrix step (slot J = t mod 16 of the output ring r, r[u mod 16] = y_u):
@@ -52,41 +52,6 @@ __device__ __forceinline__ double fwd_row_step(double a, const double (&r)[NP]) 
         "v"(r[(J - 6) & 15]), "v"(r[(J - 5) & 15]), "v"(r[(J - 4) & 15]), "v"(r[(J - 3) & 15]),
         "v"(r[(J - 2) & 15]), "v"(r[(J - 1) & 15]));
   return acc0 + acc1;
-}
-
-// Adjoint (transposed) matrix step at slot J = t mod 16 of the pending ring p
-// (p[u mod 16] = pending sum for position u, positions t .. t-15):
-//   lam = p[J];  p[J] = 0;  p[(J - i) & 15] -= A[t,i] * lam  (i = 1..16)
-// Lag 1 goes first: it feeds the next step's lam. For complex data the caller
-// passes conj(A); here the data is real.
-template <int J>
-__device__ __forceinline__ void adj_row_step(double a, double (&p)[NP]) {
-  const double lam = p[J];
-  p[J] = 0.0;
-  asm("s_nop 1\n\t"
-      "v_fmac_f64_dpp %0, -%16, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %2, -%16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %3, -%16, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %4, -%16, %17 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %5, -%16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %6, -%16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %7, -%16, %17 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %8, -%16, %17 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %9, -%16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %10, -%16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %11, -%16, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %12, -%16, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %13, -%16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %14, -%16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %15, -%16, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf"
-      : "+v"(p[(J - 1) & 15]), "+v"(p[(J - 2) & 15]), "+v"(p[(J - 3) & 15]),
-        "+v"(p[(J - 4) & 15]), "+v"(p[(J - 5) & 15]), "+v"(p[(J - 6) & 15]),
-        "+v"(p[(J - 7) & 15]), "+v"(p[(J - 8) & 15]), "+v"(p[(J - 9) & 15]),
-        "+v"(p[(J - 10) & 15]), "+v"(p[(J - 11) & 15]), "+v"(p[(J - 12) & 15]),
-        "+v"(p[(J - 13) & 15]), "+v"(p[(J - 14) & 15]), "+v"(p[(J - 15) & 15]),
-        "+v"(p[(J - 16) & 15])
-      : "v"(a), "v"(lam));
 }
 
 // Carry link on lanes 0-15 (lane r owns s_r): returns z_r + sum_k M[r][k] s_k in
@@ -163,10 +128,13 @@ __device__ __forceinline__ V fwd_quad_step(V (&R)[4], const float4 c, float x, i
 // Adjoint quad step at time t (slot J = t mod 4), descending t. State: `now` in
 // lane 0 is the pending sum of position t; lane ql's window P holds the pending
 // sums of positions [t-4-4ql, t-1-4ql] at slot u mod 4. Returns lambda_t and
-// leaves the state for step t-1.
+// leaves the state for step t-1. A step that is not `live` (past the end of the
+// signal) has lambda_t = 0 and leaves the pending sums as they are.
 template <int J, typename V>
-__device__ __forceinline__ V adj_quad_step(V (&P)[4], V& now, const float4 c, float g, int ql) {
-  const V lam = qperm<QP_LANE0>((V)g + now);
+__device__ __forceinline__ V adj_quad_step(V (&P)[4], V& now, const float4 c, float g, int ql,
+                                           bool live) {
+  const V lam0 = qperm<QP_LANE0>((V)g + now);
+  const V lam = live ? lam0 : (V)0;
   const float cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
   for (int m = 0; m < 4; ++m) P[(J - 1 - m) & 3] -= (V)cc[m] * lam;

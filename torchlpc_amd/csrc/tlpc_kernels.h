@@ -29,6 +29,7 @@ struct ChunkArgs {
   int neg;    // 1: the coefficient array holds -A (first-order scan weights w = -A)
   typename Wide<S>::type* summ;   // chunk summaries [B, P-1, NP*NP+NP]
   typename Wide<S>::type* carry;  // chunk carries [B, P, NP]
+  typename Wide<S>::type* wsum;   // order-16 adjoint: zero-state responses [B, P-1, NP]
 };
 
 template <typename S, int NP>
@@ -45,7 +46,11 @@ template <typename S>
 int launch_fill_zero(S* p, int64_t count, hipStream_t st);
 
 // Order-16 float32 chunked-scan kernels (DPP matrix rows, quad reruns).
-int launch_chunk16_fwd_summary(ChunkArgs<float> a, hipStream_t st);
+// Forward summaries: transition matrices Phi and zero-state responses z; with
+// with_z false only Phi (an adjoint that has no forward summaries to reuse).
+int launch_chunk16_fwd_summary(ChunkArgs<float> a, bool with_z, hipStream_t st);
+// Adjoint zero-state responses w (into a.wsum); the adjoint transition matrices
+// are the transposes of the forward Phi.
 int launch_chunk16_adj_summary(ChunkArgs<float> a, hipStream_t st);
 int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st);
 int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st);

@@ -13,6 +13,7 @@
 #include <c10/hip/HIPStream.h>
 #include <torch/library.h>
 
+#include <algorithm>
 #include <tuple>
 
 #include "../../include/torchlpc_amd.h"
@@ -74,8 +75,11 @@ void check_inputs(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi
   TORCH_CHECK(N >= 1, "torchlpc_amd::", op, ": filter order must be >= 1");
 }
 
-at::Tensor lpc_forward_hip(const at::Tensor& x_, const at::Tensor& A_, const at::Tensor& zi_,
-                           int64_t strategy, int64_t chunk_len) {
+// Forward; with `keep` the workspace is sized for the backward as well and
+// returned when the backward can reuse its chunk summaries (else an empty tensor).
+std::tuple<at::Tensor, at::Tensor> lpc_forward_impl(const at::Tensor& x_, const at::Tensor& A_,
+                                                    const at::Tensor& zi_, int64_t strategy,
+                                                    int64_t chunk_len, bool keep) {
   check_inputs(x_, A_, zi_, "lpc");
   c10::DeviceGuard guard(x_.device());
   const auto x = x_.contiguous();
@@ -85,19 +89,38 @@ at::Tensor lpc_forward_hip(const at::Tensor& x_, const at::Tensor& A_, const at:
   auto y = at::empty_like(x);
   const int dt = dtype_code(x);
   const int flags = ti ? TLPC_TIME_INVARIANT : 0;
-  const size_t wsb = tlpc_lpc_forward_workspace(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
-                                                (int32_t)chunk_len);
+  size_t wsb = tlpc_lpc_forward_workspace(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
+                                          (int32_t)chunk_len);
+  keep = keep && tlpc_lpc_summary_bytes(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
+                                        (int32_t)chunk_len) > 0;
+  if (keep) {
+    const int all = TLPC_NEED_DX | TLPC_NEED_DA | TLPC_NEED_DZI | flags;
+    wsb = std::max(wsb, tlpc_lpc_backward_workspace(dt, B, T, (int32_t)N, all, (int32_t)strategy,
+                                                    (int32_t)chunk_len));
+  }
   auto ws = workspace(x, wsb);
   check_rc(tlpc_lpc_forward(dt, x.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(), B, T,
                             (int32_t)N, flags, (int32_t)strategy, (int32_t)chunk_len,
                             wsb ? ws.data_ptr() : nullptr, wsb, stream_of(x)),
            "lpc forward");
-  return y;
+  return {y, keep ? ws : at::empty({0}, x.options().dtype(at::kByte))};
+}
+
+at::Tensor lpc_forward_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
+                           int64_t strategy, int64_t chunk_len) {
+  return std::get<0>(lpc_forward_impl(x, A, zi, strategy, chunk_len, false));
+}
+
+std::tuple<at::Tensor, at::Tensor> lpc_keep_hip(const at::Tensor& x, const at::Tensor& A,
+                                                const at::Tensor& zi, int64_t strategy,
+                                                int64_t chunk_len) {
+  return lpc_forward_impl(x, A, zi, strategy, chunk_len, true);
 }
 
 std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_hip(
     const at::Tensor& g_, const at::Tensor& A_, const at::Tensor& zi_, const at::Tensor& y_,
-    bool need_x, bool need_a, bool need_zi, int64_t strategy, int64_t chunk_len) {
+    bool need_x, bool need_a, bool need_zi, int64_t strategy, int64_t chunk_len,
+    const c10::optional<at::Tensor>& summary) {
   check_inputs(g_, A_, zi_, "lpc_backward");
   TORCH_CHECK(y_.sizes() == g_.sizes(), "torchlpc_amd::lpc_backward: y and grad shapes differ");
   c10::DeviceGuard guard(g_.device());
@@ -119,12 +142,25 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_hip(
   const int dt = dtype_code(g);
   const size_t wsb = tlpc_lpc_backward_workspace(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
                                                  (int32_t)chunk_len);
-  auto ws = workspace(g, wsb);
+  at::Tensor ws;
+  if (summary.has_value() && summary->numel() > 0) {
+    // the forward's workspace (lpc_keep): its leading summaries are reused
+    TORCH_CHECK(summary->device() == g.device() && summary->scalar_type() == at::kByte &&
+                    summary->is_contiguous() && (size_t)summary->numel() >= wsb &&
+                    tlpc_lpc_summary_bytes(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
+                                           (int32_t)chunk_len) > 0,
+                "torchlpc_amd::lpc_backward: `summary` does not come from lpc_keep with the same "
+                "shapes, strategy and chunk_len");
+    ws = *summary;
+    flags |= TLPC_REUSE_SUMMARY;
+  } else {
+    ws = workspace(g, wsb);
+  }
   check_rc(tlpc_lpc_backward(dt, g.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(),
                              need_x ? dx.data_ptr() : nullptr, need_a ? dA.data_ptr() : nullptr,
                              need_zi ? dzi.data_ptr() : nullptr, B, T, (int32_t)N, flags,
                              (int32_t)strategy, (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr,
-                             wsb, stream_of(g)),
+                             wsb ? (size_t)ws.numel() : 0, stream_of(g)),
            "lpc backward");
   return {dx, dA, dzi};
 }
@@ -193,11 +229,13 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> scan_backward_hip(
 at::Tensor lpc_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t, int64_t) {
   cpu_unsupported("lpc");
 }
-std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_cpu(const at::Tensor&,
-                                                                const at::Tensor&,
-                                                                const at::Tensor&,
-                                                                const at::Tensor&, bool, bool,
-                                                                bool, int64_t, int64_t) {
+std::tuple<at::Tensor, at::Tensor> lpc_keep_cpu(const at::Tensor&, const at::Tensor&,
+                                                const at::Tensor&, int64_t, int64_t) {
+  cpu_unsupported("lpc_keep");
+}
+std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_cpu(
+    const at::Tensor&, const at::Tensor&, const at::Tensor&, const at::Tensor&, bool, bool, bool,
+    int64_t, int64_t, const c10::optional<at::Tensor>&) {
   cpu_unsupported("lpc_backward");
 }
 at::Tensor scan_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t) {
@@ -216,8 +254,12 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> scan_backward_cpu(const at::Tenso
 TORCH_LIBRARY(torchlpc_amd, m) {
   m.def("lpc(Tensor x, Tensor A, Tensor zi, int strategy=0, int chunk_len=0) -> Tensor");
   m.def(
+      "lpc_keep(Tensor x, Tensor A, Tensor zi, int strategy=0, int chunk_len=0) -> "
+      "(Tensor, Tensor)");
+  m.def(
       "lpc_backward(Tensor grad_y, Tensor A, Tensor zi, Tensor y, bool need_x, bool need_a, "
-      "bool need_zi, int strategy=0, int chunk_len=0) -> (Tensor, Tensor, Tensor)");
+      "bool need_zi, int strategy=0, int chunk_len=0, Tensor? summary=None) -> "
+      "(Tensor, Tensor, Tensor)");
   m.def("scan(Tensor x, Tensor w, Tensor init, int chunk_len=0) -> Tensor");
   m.def(
       "scan_backward(Tensor grad, Tensor w, Tensor init, Tensor h, bool need_x, bool need_w, "
@@ -226,6 +268,7 @@ TORCH_LIBRARY(torchlpc_amd, m) {
 
 TORCH_LIBRARY_IMPL(torchlpc_amd, CUDA, m) {
   m.impl("lpc", &lpc_forward_hip);
+  m.impl("lpc_keep", &lpc_keep_hip);
   m.impl("lpc_backward", &lpc_backward_hip);
   m.impl("scan", &scan_hip);
   m.impl("scan_backward", &scan_backward_hip);
@@ -233,6 +276,7 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CUDA, m) {
 
 TORCH_LIBRARY_IMPL(torchlpc_amd, CPU, m) {
   m.impl("lpc", &lpc_cpu);
+  m.impl("lpc_keep", &lpc_keep_cpu);
   m.impl("lpc_backward", &lpc_backward_cpu);
   m.impl("scan", &scan_cpu);
   m.impl("scan_backward", &scan_backward_cpu);
