@@ -14,6 +14,14 @@ dominant op measured with HIP events on its stream, the plain GPU form
 timed on this host's cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-plain]
+  python bench.py --trace-summary <rocprofv3 results.db>
+  python bench.py --pmc-summary <FETCH_SIZE run dir> <WRITE_SIZE run dir>
+
+"Dominant kernel" in the roofline object is the dominant op of the step (the
+backward at this shape): its algorithmic bytes per call over its event-timed
+duration. The op is a short fixed sequence of kernels (OP_KERNELS); the rocprof
+trace summary adds their average durations for comparison, and `traffic` is
+their counter-measured HBM bytes per call from the committed summary.
 """
 from __future__ import annotations
 
@@ -25,6 +33,14 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+COPY_BYTES = 1 << 30  # the --calibrate-copy kernel reads and writes this many bytes
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+# kernels of one forward and one backward call at the order-16 float32 shape
+OP_KERNELS = {
+    "lpc (forward)": ("sum_fwd<true>", "carry16<true>", "rerun_fwd"),
+    "lpc_backward (fused adjoint)": ("sum_adj", "carry16<false>", "rerun_adj<true>"),
+}
 
 
 def log(*a):
@@ -59,16 +75,18 @@ def make_inputs(B, T, N, device, seed=1002, hop=240, kmax=0.9, zi_std=0.1):
     return x.to(f32), a.to(f32).contiguous(), zi.to(f32), w.to(f32)
 
 
-def cpu_baseline(T, N, sample_signals=None, reps=3):
+def cpu_baseline(T, N, sample_signals=64, min_seconds=10.0):
     """Reference CPU path (torchlpc/csrc/scan_cpu.cpp op + the core.py:186-229 backward
-    composition), or the C restatement when the reference op is not built."""
+    composition), or the C restatement when the reference op is not built. The
+    sample is `sample_signals` signals of the workload, run repeatedly for about
+    `min_seconds` of CPU time (median run)."""
     import numpy as np
     import torch
 
     from oracle import oracle as O
 
     cores = os.cpu_count() or 1
-    Bs = sample_signals or max(2, cores)
+    Bs = sample_signals
     rng = np.random.default_rng(7)
     x, A, zi, w = O.make_reflection_inputs(Bs, T, N, 240, 7)
     x, A, zi, w = (v.astype(np.float32) for v in (x, A, zi, w))
@@ -95,11 +113,13 @@ def cpu_baseline(T, N, sample_signals=None, reps=3):
     del rng
     run()
     times = []
-    for _ in range(reps):
+    t_all = time.perf_counter()
+    while len(times) < 3 or time.perf_counter() - t_all < min_seconds:
         t0 = time.perf_counter()
         run()
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
+    reps = len(times)
     return {
         "value": Bs * T / t / 1e6,
         "unit": "Msamples/s",
@@ -110,6 +130,85 @@ def cpu_baseline(T, N, sample_signals=None, reps=3):
         f"{min(threads, Bs)}; host os.cpu_count()={cores})",
         "seconds_per_run": t,
     }
+
+
+def _short(name):
+    """Kernel name without namespace, return type and argument list."""
+    name = name.split("(")[0].replace("void ", "").strip()
+    return name.split("::")[-1]
+
+
+def trace_summary(db_path):
+    """Per-kernel average durations of a rocprofv3 --kernel-trace database, and the
+    per-call total of the kernels of each op."""
+    import sqlite3
+
+    con = sqlite3.connect(db_path)
+    rows = con.execute(
+        "select name, count(*), avg(duration) from kernels group by name order by sum(duration) desc"
+    ).fetchall()
+    lines = [f"{'kernel':40s} {'calls':>5s} {'avg_us':>10s}"]
+    avg = {}
+    for name, n, d in rows:
+        k = _short(name)
+        avg[k] = d / 1e3
+        if "tlpc" in name or k.startswith("neg"):
+            lines.append(f"{k:40s} {n:5d} {d / 1e3:10.1f}")
+    for op, ks in OP_KERNELS.items():
+        if all(k in avg for k in ks):
+            lines.append(f"op {op}: sum of kernel averages = {sum(avg[k] for k in ks):.1f} us " f"({' + '.join(ks)})")
+    return "\n".join(lines)
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel, scaled by what the same
+    counters read for the --calibrate-copy kernel (COPY_BYTES read and written)."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == counter:
+                    vals.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in vals.items()}
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy = next((k for k in fetch if "neg" in k), None)
+    if copy is None:
+        raise SystemExit("no --calibrate-copy kernel in the counter runs")
+    ff = COPY_BYTES / (fetch[copy] * 1024)
+    fw = COPY_BYTES / (write[copy] * 1024)
+    lines = [
+        f"calibration kernel {copy}: FETCH_SIZE {fetch[copy]:.0f} KiB, WRITE_SIZE {write[copy]:.0f} KiB "
+        f"for {COPY_BYTES} bytes each -> scale fetch x{ff:.3f}, write x{fw:.3f}",
+        f"{'kernel':40s} {'FETCH_KiB':>12s} {'WRITE_KiB':>12s} {'traffic_MB':>11s}",
+    ]
+    traffic = {}
+    for k in sorted(fetch):
+        if k not in write:
+            continue
+        t = fetch[k] * 1024 * ff + write[k] * 1024 * fw
+        traffic[k] = t
+        if any(k in ks for ks in OP_KERNELS.values()):
+            lines.append(f"{k:40s} {fetch[k]:12.0f} {write[k]:12.0f} {t / 1e6:11.1f}")
+    for op, ks in OP_KERNELS.items():
+        if all(k in traffic for k in ks):
+            lines.append(f"traffic_bytes_per_call {op}: {sum(traffic[k] for k in ks):.0f}")
+    return "\n".join(lines)
+
+
+def committed_traffic(op, B, T, N):
+    """HBM bytes per call of `op` from the committed counter summary, if there is
+    one for this shape."""
+    path = os.path.join(PROFILES, f"lpc_traffic_B{B}_T{T}_N{N}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_call {op}:"):
+            return float(line.rsplit(":", 1)[1])
+    return None
 
 
 def main():
@@ -124,7 +223,18 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, run one copy-like kernel of known size (for counter scaling)")
+    ap.add_argument("--trace-summary", metavar="DB", help="summarise a rocprofv3 kernel-trace database and exit")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise rocprofv3 FETCH_SIZE / WRITE_SIZE runs and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
+    if args.pmc_summary:
+        print(pmc_summary(*args.pmc_summary))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(
@@ -233,10 +343,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": committed_traffic(dom, B, T, N),
             "algorithmic_bytes": dom_bytes,
+            "kernels": list(OP_KERNELS.get(dom, ())),
         },
     }
+
+    if args.calibrate_copy:
+        src = torch.ones(COPY_BYTES // 4, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     if not args.no_plain:
         log("[bench] plain GPU form (forward kernel + unfused reference-style backward)")
