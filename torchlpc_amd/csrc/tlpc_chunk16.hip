@@ -42,6 +42,20 @@ __device__ __forceinline__ float4 ld_row(const float* Ab, int t, int ql) {
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// The prologue fills of the prefetch rings pass through an empty asm statement.
+// A ring slot is a loop phi whose incoming values are both loads (the prologue's
+// and the previous iteration's); without the barrier InstCombine turns that phi
+// of loads into one load at the loop header, issued right before its use, and
+// the prefetch is gone.
+__device__ __forceinline__ float opaque(float v) {
+  asm("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ float4 opaque(float4 v) {
+  asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+  return v;
+}
+
 // y_u of one signal, with the initial conditions y_{-1-k} = zi[k] (k < 16) and
 // zero elsewhere before 0: the address is always valid (indices past T clamp to
 // T-1, a value the callers never use) and yok says whether the value counts. The
@@ -125,8 +139,8 @@ __device__ __forceinline__ void fwd_quad_run(V (&R)[4], const FwdIo& io, int t0,
 #pragma unroll
   for (int j = 0; j < PF; ++j) {
     const int tj = min(t0 + j, io.T - 1);
-    cq[j] = ld_row(io.Ab, tj, ql);
-    xq[j] = io.xb[tj];
+    cq[j] = opaque(ld_row(io.Ab, tj, ql));
+    xq[j] = opaque(io.xb[tj]);
   }
   for (int jb = 0; jb < len; jb += 16) {
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -210,9 +224,9 @@ __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const 
 #pragma unroll
   for (int j = 0; j < PF; ++j) {
     const int tc = clampi(tstart - j, 0, io.T - 1);
-    cq[j] = ld_row(io.Ab, tc, ql);
-    gq[j] = io.gb[tc];
-    yq[j] = DA ? *yaddr(io.yb, io.zib, tstart - j - 17, io.T) : 0.f;
+    cq[j] = opaque(ld_row(io.Ab, tc, ql));
+    gq[j] = opaque(io.gb[tc]);
+    yq[j] = DA ? opaque(*yaddr(io.yb, io.zib, tstart - j - 17, io.T)) : 0.f;
   }
   for (int tb = t0 + L - 16; tb >= t0; tb -= 16) {
     float4 dxo = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -247,7 +261,7 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
     for (int i = 0; i < NP; ++i) r[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // y_{-1-kk} = 1
     float cf[NP];
 #pragma unroll
-    for (int j = 0; j < RD; ++j) cf[j] = ap[j * NP];
+    for (int j = 0; j < RD; ++j) cf[j] = opaque(ap[j * NP]);
     for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap, jb, jlim);
     double* out = a.summ + cid * (NP * NP + NP);
 #pragma unroll
