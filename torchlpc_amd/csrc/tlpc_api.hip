@@ -114,7 +114,7 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   const size_t carry = (size_t)B * p.P * p.np * ws_es;
   const size_t dapart =
       (backward && (flags & TLPC_TIME_INVARIANT) && (flags & TLPC_NEED_DA) && !(flags & TLPC_DA_FULL))
-          ? (size_t)B * p.P * p.np * es
+          ? (size_t)B * p.P * p.np * ws_es
           : 0;
   p.c16 = p.strategy == TLPC_STRATEGY_CHUNKED && chunk16_ok(dtype, n, flags, neg, p.L, T);
   // the order-16 adjoint keeps its zero-state responses apart from the summaries,
@@ -211,7 +211,7 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   a.carry = reinterpret_cast<W*>(w + p.carry_off);
   a.wsum = reinterpret_cast<W*>(w + p.wsum_off);
   if (need_da && da_reduced) {
-    a.dapart = reinterpret_cast<S*>(w + p.dapart_off);
+    a.dapart = reinterpret_cast<W*>(w + p.dapart_off);
     a.da_ti = static_cast<S*>(dA);
   } else if (need_da) {
     a.dA = static_cast<S*>(dA);

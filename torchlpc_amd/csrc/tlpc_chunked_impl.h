@@ -373,8 +373,11 @@ __device__ __forceinline__ void load_row(S (&row)[NP], const ChunkArgs<S>& a, co
 }
 
 // Phase 3 (forward): one lane per chunk re-filters from its carry and writes y.
+// The recurrence runs in the wide type (float64 for float32 data), so the only
+// single-precision rounding of an output is its final store.
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cid >= a.B * a.P) return;
   const int64_t b = cid / a.P;
@@ -382,10 +385,10 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   const int64_t t0 = c * a.L;
   const int64_t rem = a.T - t0;
   const int len = rem < a.L ? (int)rem : a.L;
-  S ring[NP];
-  const typename Wide<S>::type* cr = a.carry + cid * NP;
+  W ring[NP];
+  const W* cr = a.carry + cid * NP;
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = convert<S>(cr[r]);
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
   const S* xb = a.x + b * a.T;
   const S* Ab = a.A + b * a.a_bstride;
   S* yb = a.y + b * a.T;
@@ -396,11 +399,12 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
         const int64_t t = t0 + jb + j;
         S row[NP];
         load_row<S, NP>(row, a, Ab + t * a.a_tstride);
-        S acc = xb[t];
+        W acc = convert<W>(xb[t]);
 #pragma unroll
-        for (int i = NP; i >= 1; --i) acc = Num<S>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
+        for (int i = NP; i >= 1; --i)
+          acc = Num<W>::fms(acc, convert<W>(row[i - 1]), ring[(j - i) & (NP - 1)]);
         ring[j] = acc;
-        yb[t] = acc;
+        yb[t] = convert<S>(acc);
       }
     }
   }
@@ -408,18 +412,19 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
 
 // Phase 3 (adjoint): one lane per chunk walks its samples backwards from the carry,
 // writes dx and dA (time-varying) or a per-chunk partial of da (time-invariant),
-// and chunk 0 writes dzi.
+// and chunk 0 writes dzi. Arithmetic in the wide type, as in the forward rerun.
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cid >= a.B * a.P) return;
   const int64_t b = cid / a.P;
   const int64_t c = cid - b * a.P;
   const int64_t t0 = c * a.L;
-  S ring[NP];  // pending sums, slot u mod NP for position u
-  const typename Wide<S>::type* cr = a.carry + cid * NP;
+  W ring[NP];  // pending sums, slot u mod NP for position u
+  const W* cr = a.carry + cid * NP;
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = convert<S>(cr[r]);
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
   const S* gb = a.x + b * a.T;  // upstream gradient
   const S* Ab = a.A + b * a.a_bstride;
   const S* yb = a.y + b * a.T;
@@ -433,15 +438,16 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
     const int64_t k = -u - 1;
     return (zib && k < a.n) ? zib[k] : Num<S>::zero();
   };
-  S yr[NP];
+  W yr[NP];
   const int64_t tstart = t0 + a.L - 1;  // first processed (possibly virtual) time
   if (need_da) {
 #pragma unroll
-    for (int i = 1; i <= NP; ++i) yr[(NP - 1 - i + NP) & (NP - 1)] = Num<S>::conj(yval(tstart - i));
+    for (int i = 1; i <= NP; ++i)
+      yr[(NP - 1 - i + NP) & (NP - 1)] = convert<W>(Num<S>::conj(yval(tstart - i)));
   }
-  S dacc[NP];
+  W dacc[NP];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) dacc[i] = Num<S>::zero();
+  for (int i = 0; i < NP; ++i) dacc[i] = Num<W>::zero();
 
   for (int jb = 0; jb < a.L; jb += NP) {
 #pragma unroll
@@ -451,54 +457,56 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
       if (t < a.T) {
         S row[NP];
         load_row<S, NP>(row, a, Ab + t * a.a_tstride);
-        const S lam = gb[t] + ring[j];
-        ring[j] = Num<S>::zero();
+        const W lam = convert<W>(gb[t]) + ring[j];
+        ring[j] = Num<W>::zero();
 #pragma unroll
         for (int i = 1; i <= NP; ++i)
-          ring[(j - i) & (NP - 1)] = Num<S>::fms(ring[(j - i) & (NP - 1)], Num<S>::conj(row[i - 1]), lam);
-        if (dxb) dxb[t] = lam;
+          ring[(j - i) & (NP - 1)] =
+              Num<W>::fms(ring[(j - i) & (NP - 1)], Num<W>::conj(convert<W>(row[i - 1])), lam);
+        if (dxb) dxb[t] = convert<S>(lam);
         if (need_da) {
           if (dAb) {
             S* drow = dAb + t * a.n;
 #pragma unroll
             for (int i = 1; i <= NP; ++i)
-              if (i <= a.n) drow[i - 1] = coef(a, -(lam * yr[(j - i) & (NP - 1)]));
+              if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * yr[(j - i) & (NP - 1)])));
           } else {
 #pragma unroll
-            for (int i = 1; i <= NP; ++i) dacc[i - 1] = Num<S>::fms(dacc[i - 1], lam, yr[(j - i) & (NP - 1)]);
+            for (int i = 1; i <= NP; ++i) dacc[i - 1] = Num<W>::fms(dacc[i - 1], lam, yr[(j - i) & (NP - 1)]);
           }
         }
       }
       if (need_da) {
         // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
-        yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval(t - 1 - NP));
+        yr[(j - 1) & (NP - 1)] = convert<W>(Num<S>::conj(yval(t - 1 - NP)));
       }
     }
   }
   if (a.dapart) {
-    S* dp = a.dapart + cid * NP;
+    W* dp = a.dapart + cid * NP;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) dp[i] = coef(a, dacc[i]);
+    for (int i = 0; i < NP; ++i) dp[i] = a.neg ? -dacc[i] : dacc[i];
   }
   if (c == 0 && a.dzi) {
     // pending sums for positions -1-r sit in slots NP-1-r
 #pragma unroll
     for (int r = 0; r < NP; ++r)
-      if (r < a.n) a.dzi[b * a.n + r] = ring[NP - 1 - r];
+      if (r < a.n) a.dzi[b * a.n + r] = convert<S>(ring[NP - 1 - r]);
   }
 }
 
-// Time-invariant da: deterministic sum of the per-chunk partials.
+// Time-invariant da: deterministic sum of the per-chunk partials (wide type).
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void ti_da_reduce(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.B * a.n) return;
   const int64_t b = idx / a.n;
   const int i = (int)(idx - b * a.n);
-  S acc = Num<S>::zero();
-  const S* dp = a.dapart + b * a.P * NP + i;
+  W acc = Num<W>::zero();
+  const W* dp = a.dapart + b * a.P * NP + i;
   for (int64_t c = 0; c < a.P; ++c) acc = acc + dp[c * NP];
-  a.da_ti[b * a.n + i] = acc;
+  a.da_ti[b * a.n + i] = convert<S>(acc);
 }
 
 // ----------------------------------------------------------------------------

@@ -19,7 +19,7 @@ struct ChunkArgs {
   S* dx;
   S* dA;      // time-varying dA [B,T,n] (adjoint) or null
   S* dzi;     // [B,n] or null
-  S* dapart;  // time-invariant da: per-chunk partial sums [B,P,NP] (workspace) or null
+  typename Wide<S>::type* dapart;  // time-invariant da: per-chunk partials [B,P,NP] (workspace) or null
   S* da_ti;   // time-invariant da output [B,n] or null
   int64_t B, T;
   int n;      // filter order
