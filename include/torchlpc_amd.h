@@ -109,6 +109,36 @@ size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32_t N, int32_
                               int32_t strategy, int32_t chunk_len);
 
 /*
+ * Ragged batches: K signals of one order N (N <= 32) with their own lengths,
+ * packed back to back. No reference counterpart: the reference takes only
+ * rectangular [B, T] batches (torchlpc/__init__.py:63-104); these entry points
+ * give the same per-signal results as tlpc_lpc_forward / tlpc_lpc_backward on
+ * each signal alone. Signal k reads x[x_off[k] .. + lengths[k]) (also the
+ * offset of y, g, dx), A[a_off[k] .. + lengths[k]*N) as [lengths[k], N] rows
+ * (also dA), zi[z_off[k] .. + N) (also dzi); offsets are in elements.
+ *
+ * tlpc_lpc_ragged_plan (host only) chooses the chunk length (chunk_len 0: the
+ * library's choice) and fills `table` (int64, table_len entries; pass NULL to
+ * query table_len, G and the chunk length first). The caller copies the table
+ * to device memory and passes it, with K, G, max_chunks and the chunk length,
+ * to the launches. Time-varying coefficients only.
+ */
+int tlpc_lpc_ragged_plan(int64_t K, const int64_t* x_off, const int64_t* a_off,
+                         const int64_t* z_off, const int64_t* lengths, int32_t N, int32_t dtype,
+                         int32_t chunk_len, int64_t* table, int64_t* table_len, int64_t* G,
+                         int64_t* max_chunks, int32_t* chunk_len_out);
+size_t tlpc_lpc_ragged_workspace(int dtype, int64_t G, int32_t N);
+int tlpc_lpc_ragged_forward(int dtype, const void* x, const void* A, const void* zi, void* y,
+                            const int64_t* table, int64_t K, int64_t G, int64_t max_chunks,
+                            int32_t N, int32_t chunk_len, void* workspace, size_t workspace_bytes,
+                            void* hip_stream);
+int tlpc_lpc_ragged_backward(int dtype, const void* g, const void* A, const void* zi,
+                             const void* y, void* dx, void* dA, void* dzi, const int64_t* table,
+                             int64_t K, int64_t G, int64_t max_chunks, int32_t N, int32_t flags,
+                             int32_t chunk_len, void* workspace, size_t workspace_bytes,
+                             void* hip_stream);
+
+/*
  * First-order recurrence h[t] = w[t] * h[t-1] + x[t], h[-1] = init[b] (the order-1
  * filter with A = -w). Replaces the reference op torchlpc::scan(input, weights,
  * initials) (torchlpc/csrc/scan_cpu.cpp:28-78,124-141, schema :168;

@@ -1,0 +1,119 @@
+"""Ragged batches (packed signals with their own lengths and orders) against the
+FP64 oracle run on each signal alone, including the full 512-signal workload
+(lengths ~ U{1024..65536}, orders in {2, 4, 8, 16, 32}) and a HIP-graph capture
+of its forward + backward."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.gpu_util import REL_TOL, dev, host, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _pack(sigs, dtype):
+    x = np.concatenate([s[0] for s in sigs]).astype(dtype)
+    a = np.concatenate([s[1].ravel() for s in sigs]).astype(dtype)
+    zi = np.concatenate([s[2] for s in sigs]).astype(dtype)
+    g = np.concatenate([s[3] for s in sigs]).astype(dtype)
+    return x, a, zi, g
+
+
+def _check(sigs, dtype, plan, y, dx, da, dzi, tol):
+    xs = plan.split(torch.from_numpy(np.arange(plan.total_samples)), "x")
+    worst = 0.0
+    yo, dxo, dao, dzo = host(y), host(dx), host(da), host(dzi)
+    for k, (x, A, zi, g) in enumerate(sigs):
+        up = lambda v: np.asarray(v).astype(dtype).astype(np.complex128 if np.iscomplexobj(v) else np.float64)  # noqa: E731
+        X, AA, Z, G = up(x)[None], up(A)[None], up(zi)[None], up(g)[None]
+        y64 = O.lpc(X, AA, Z)
+        rx, rA, rzi = O.lpc_backward(G, AA, Z, y64)
+        s0, s1 = int(plan.x_off[k]), int(plan.x_off[k + 1])
+        a0, a1 = int(plan.a_off[k]), int(plan.a_off[k + 1])
+        z0, z1 = int(plan.z_off[k]), int(plan.z_off[k + 1])
+        errs = [
+            rel(yo[s0:s1], y64[0]),
+            rel(dxo[s0:s1], rx[0]),
+            rel(dao[a0:a1], rA[0].ravel()),
+            rel(dzo[z0:z1], rzi[0]),
+        ]
+        worst = max(worst, *errs)
+        assert max(errs) <= tol, (k, len(x), A.shape[1], errs)
+    del xs
+    return worst
+
+
+def _mixed(rng, lengths, orders, cplx=False):
+    sigs = []
+    for T, N in zip(lengths, orders):
+        x = rng.standard_normal(T)
+        kmax = min(0.8, 4.0 / N)
+        k = O._interp_anchors(rng.uniform(-kmax, kmax, (1, (T - 1) // 64 + 2, N)), T, 64)
+        A = O.step_up(k)[0]
+        zi = rng.standard_normal(N)
+        g = rng.standard_normal(T)
+        if cplx:
+            x = x + 1j * rng.standard_normal(T)
+            A = A * np.exp(1j * rng.uniform(-0.2, 0.2, A.shape))
+            zi = zi + 1j * rng.standard_normal(N)
+            g = g + 1j * rng.standard_normal(T)
+        sigs.append((x, A, zi, g))
+    return sigs
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64])
+def test_ragged_mixed_lengths_and_orders(dtype):
+    from torchlpc_amd import RaggedPlan, ragged_lpc
+
+    rng = np.random.default_rng(11)
+    lengths = [1, 15, 16, 17, 33, 100, 255, 256, 257, 1000, 3001, 4096, 40, 7, 2048, 999]
+    orders = [1, 2, 3, 4, 8, 16, 17, 32, 2, 16, 32, 4, 1, 16, 8, 3]
+    sigs = _mixed(rng, lengths, orders, np.issubdtype(dtype, np.complexfloating))
+    x, a, zi, g = _pack(sigs, dtype)
+    X, AA, Z = (dev(v).requires_grad_() for v in (x, a, zi))
+    plan = RaggedPlan(lengths, orders, X.device, X.dtype)
+    y = ragged_lpc(X, AA, plan, Z)
+    y.backward(dev(g))
+    _check(sigs, dtype, plan, y, X.grad, AA.grad, Z.grad, REL_TOL[dtype])
+
+
+def test_ragged_full_workload_and_graph_replay():
+    from torchlpc_amd import RaggedPlan
+    from torchlpc_amd.ragged import RaggedLPC
+
+    sigs = O.make_config4()
+    lengths = [len(s[0]) for s in sigs]
+    orders = [s[1].shape[1] for s in sigs]
+    x, a, zi, g = _pack(sigs, np.float32)
+    X, AA, Z, Gd = dev(x), dev(a), dev(zi), dev(g)
+    plan = RaggedPlan(lengths, orders, X.device, X.dtype)
+    ops = torch.ops.torchlpc_amd
+
+    def step():
+        y = RaggedLPC.forward(X, AA, Z, plan)
+        dx, da, dzi = torch.empty_like(X), torch.empty_like(AA), torch.empty_like(Z)
+        for n, table, K, G, maxp, L in plan.groups:
+            ops.lpc_ragged_backward_(Gd, AA, Z, y, table, dx, da, dzi, K, G, maxp, n, L, True, True, True)
+        return y, dx, da, dzi
+
+    eager = step()
+    worst = _check(sigs, np.float32, plan, *eager, 1e-5)
+    print("ragged workload worst rel err", worst)
+    eager = [t.clone() for t in eager]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = step()
+    for r in range(3):
+        torch.randn(1 << 20, device="cuda").mul_(3.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        for e, o in zip(eager, outs):
+            assert torch.equal(e, o), r

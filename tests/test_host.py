@@ -135,3 +135,25 @@ def test_meta_order16_keeps_forward_summaries():
     # nothing to keep for float64 or a time-invariant a[B, N]
     assert _ext.keep_workspace_bytes(meta(2, 5000, dtype=torch.float64), meta(2, 5000, 16, dtype=torch.float64), 0, 0) == 0
     assert _ext.keep_workspace_bytes(x, meta(2, 16), 0, 0) == 0
+
+
+def test_ragged_plan_table_layout():
+    """Host plan of a ragged batch: per order, offsets / lengths / first chunk ids,
+    then the signal of every chunk (the layout documented in the C header)."""
+    from torchlpc_amd import RaggedPlan
+
+    lengths, orders = [100, 37, 5000, 16, 1], [2, 16, 2, 32, 2]
+    p = RaggedPlan(lengths, orders, "cpu")
+    assert (p.total_samples, p.total_coeffs, p.total_state) == (5154, 11306, 54)
+    n, table, K, G, maxp, L = p.groups[0]  # order 2: signals 0, 2, 4
+    assert (n, K) == (2, 3) and L % 16 == 0
+    P = [-(-T // L) for T in (100, 5000, 1)]
+    assert G == sum(P) and maxp == max(P)
+    t = table.tolist()
+    assert t[0:3] == [0, 137, 5153]  # sample offsets
+    assert t[3:6] == [0, 792, 11304]  # coefficient offsets (T_k * N_k rows before)
+    assert t[6:9] == [0, 18, 52]  # zi offsets
+    assert t[9:12] == [100, 5000, 1]
+    assert t[12:16] == [0, P[0], P[0] + P[1], G]
+    assert t[16:] == [0] * P[0] + [1] * P[1] + [2] * P[2]
+    assert [g[0] for g in p.groups] == [2, 16, 32]

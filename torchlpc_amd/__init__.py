@@ -16,9 +16,19 @@ import torch
 
 from ._ext import EXTENSION_LOADED, LOAD_ERROR, hip_runtimes_mapped  # noqa: F401
 from .core import LPC
+from .ragged import RaggedLPC, RaggedPlan, ragged_lpc
 from .recurrence import Recurrence
 
-__all__ = ["sample_wise_lpc", "linear_recurrence", "LPC", "Recurrence", "EXTENSION_LOADED"]
+__all__ = [
+    "sample_wise_lpc",
+    "linear_recurrence",
+    "LPC",
+    "Recurrence",
+    "RaggedPlan",
+    "RaggedLPC",
+    "ragged_lpc",
+    "EXTENSION_LOADED",
+]
 __version__ = "0.8.0+mi355x"
 
 

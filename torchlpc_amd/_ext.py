@@ -14,7 +14,16 @@ PKG = Path(__file__).resolve().parent
 OPS_LIB = PKG / "_C.so"
 HIP_LIB = PKG / "libtorchlpc_amd_hip.so"
 
-_OPS = ("lpc", "lpc_keep", "lpc_backward", "scan", "scan_backward")
+_OPS = (
+    "lpc",
+    "lpc_keep",
+    "lpc_backward",
+    "lpc_ragged_plan",
+    "lpc_ragged_",
+    "lpc_ragged_backward_",
+    "scan",
+    "scan_backward",
+)
 
 EXTENSION_LOADED = False
 LOAD_ERROR: str | None = None
@@ -114,6 +123,14 @@ def _register_fake() -> None:
             dA = g.new_empty((0,))
         dzi = g.new_empty((B, N)) if need_zi else g.new_empty((0,))
         return dx, dA, dzi
+
+    @torch.library.register_fake(lib + "lpc_ragged_")
+    def _lpc_ragged_fake(x, A, zi, table, y, K, G, max_chunks, N, chunk_len):
+        return None
+
+    @torch.library.register_fake(lib + "lpc_ragged_backward_")
+    def _lpc_ragged_bwd_fake(g, A, zi, y, table, dx, dA, dzi, K, G, max_chunks, N, chunk_len, nx, na, nz):
+        return None
 
     @torch.library.register_fake(lib + "scan")
     def _scan_fake(x, w, init, chunk_len=0):

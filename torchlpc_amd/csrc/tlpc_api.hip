@@ -129,6 +129,11 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   return p;
 }
 
+// Ragged workspace: summaries and carries indexed by chunk id.
+inline size_t ragged_workspace(size_t wsz, int64_t G, int np) {
+  return align_up((size_t)G * (np * np + np) * wsz) + align_up((size_t)G * np * wsz);
+}
+
 int check_common(int dtype, int64_t B, int64_t T, int32_t n) {
   if (elem_size(dtype) == 0) return TLPC_EDTYPE;
   if (B < 0 || T < 0 || n < 1) return TLPC_EINVAL;
@@ -234,6 +239,69 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   return dispatch_np<S>(p, a, backward, st);
 }
 
+// Ragged batch of one order: the generic chunked kernels driven by the device
+// table `rg` (layout in tlpc_kernels.h). Time-varying coefficients only.
+template <typename S>
+int run_ragged(const void* x, const void* A, const void* zi, void* y, void* dx, void* dA,
+               void* dzi, const int64_t* rg, int64_t K, int64_t G, int64_t maxP, int32_t n,
+               int32_t flags, int32_t L, void* ws, size_t ws_bytes, void* stream, bool backward) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int np = compiled_order(n);
+  if (np == 0) return TLPC_EORDER;
+  if (K < 0 || G < 0 || maxP < 0 || L <= 0 || L % chunk_block(np) != 0) return TLPC_EINVAL;
+  if (K == 0) return TLPC_OK;
+  const size_t need = ragged_workspace(sizeof(typename Wide<S>::type), G, np);
+  if (ws_bytes < need || !ws) return TLPC_EWORKSPACE;
+  const bool need_dx = backward && (flags & TLPC_NEED_DX);
+  const bool need_da = backward && (flags & TLPC_NEED_DA);
+  const bool need_dzi = backward && (flags & TLPC_NEED_DZI);
+  if (!rg || !A || !x || (!backward && !y) || (need_da && !y)) return TLPC_EINVAL;
+  if ((need_dx && !dx) || (need_da && !dA) || (need_dzi && !dzi)) return TLPC_EINVAL;
+  ChunkArgs<S> a{};
+  a.x = static_cast<const S*>(x);
+  a.A = static_cast<const S*>(A);
+  a.zi = static_cast<const S*>(zi);
+  a.y = static_cast<S*>(y);
+  a.dx = need_dx ? static_cast<S*>(dx) : nullptr;
+  a.dA = need_da ? static_cast<S*>(dA) : nullptr;
+  a.dzi = need_dzi ? static_cast<S*>(dzi) : nullptr;
+  a.B = K;
+  a.T = 0;
+  a.n = n;
+  a.L = L;
+  a.P = maxP;
+  a.a_bstride = 0;
+  a.a_tstride = n;
+  a.rg = rg;
+  a.G = G;
+  using W = typename Wide<S>::type;
+  char* w = static_cast<char*>(ws);
+  a.summ = reinterpret_cast<W*>(w);
+  a.carry = reinterpret_cast<W*>(w + align_up((size_t)G * (np * np + np) * sizeof(W)));
+  if (G == 0) return TLPC_OK;
+  return dispatch_np<S>(Plan{np, TLPC_STRATEGY_CHUNKED, L, maxP}, a, backward, st);
+}
+
+int run_ragged_dtype(int dtype, const void* x, const void* A, const void* zi, void* y, void* dx,
+                     void* dA, void* dzi, const int64_t* rg, int64_t K, int64_t G, int64_t maxP,
+                     int32_t n, int32_t flags, int32_t L, void* ws, size_t ws_bytes, void* stream,
+                     bool backward) {
+  const int rc = check_common(dtype, K, 1, n);
+  if (rc != TLPC_OK) return rc;
+  switch (dtype) {
+    case TLPC_F32:
+      return run_ragged<float>(x, A, zi, y, dx, dA, dzi, rg, K, G, maxP, n, flags, L, ws, ws_bytes, stream, backward);
+    case TLPC_F64:
+      return run_ragged<double>(x, A, zi, y, dx, dA, dzi, rg, K, G, maxP, n, flags, L, ws, ws_bytes, stream, backward);
+    case TLPC_C64:
+      return run_ragged<Cx<float>>(x, A, zi, y, dx, dA, dzi, rg, K, G, maxP, n, flags, L, ws, ws_bytes, stream, backward);
+    case TLPC_C128:
+      return run_ragged<Cx<double>>(x, A, zi, y, dx, dA, dzi, rg, K, G, maxP, n, flags, L, ws, ws_bytes, stream, backward);
+    default:
+      return TLPC_EDTYPE;
+  }
+}
+
 int run_dtype(int dtype, const void* x, const void* A, const void* zi, void* y, void* dx,
               void* dA, void* dzi, int64_t B, int64_t T, int32_t n, int32_t flags,
               int32_t strategy, int32_t chunk_len, void* ws, size_t ws_bytes, void* stream,
@@ -316,6 +384,75 @@ TLPC_EXPORT size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32
   const Plan b = make_plan(dtype, B, T, N, flags, strategy, chunk_len, true);
   if (!f.c16 || !b.c16 || f.L != b.L || f.summ_bytes != b.summ_bytes) return 0;
   return f.summ_bytes;
+}
+
+TLPC_EXPORT int tlpc_lpc_ragged_plan(int64_t K, const int64_t* x_off, const int64_t* a_off,
+                                     const int64_t* z_off, const int64_t* lengths, int32_t N,
+                                     int32_t dtype, int32_t chunk_len, int64_t* table,
+                                     int64_t* table_len, int64_t* G_out, int64_t* max_chunks,
+                                     int32_t* chunk_len_out) {
+  if (K < 0 || N < 1 || !lengths || !table_len || !G_out || !max_chunks || !chunk_len_out)
+    return TLPC_EINVAL;
+  const int np = compiled_order(N);
+  if (np == 0) return TLPC_EORDER;
+  int64_t total = 0;
+  for (int64_t k = 0; k < K; ++k) {
+    if (lengths[k] < 1) return TLPC_EINVAL;
+    total += lengths[k];
+  }
+  const bool single = dtype == TLPC_F32 || dtype == TLPC_C64;
+  const int L = pick_chunk(K > 0 ? K : 1, K > 0 ? (total + K - 1) / K : 1, np, chunk_len, single);
+  int64_t G = 0, maxP = 0;
+  for (int64_t k = 0; k < K; ++k) {
+    const int64_t P = (lengths[k] + L - 1) / L;
+    G += P;
+    maxP = P > maxP ? P : maxP;
+  }
+  *G_out = G;
+  *max_chunks = maxP;
+  *chunk_len_out = L;
+  *table_len = 5 * K + 1 + G;
+  if (!table) return TLPC_OK;
+  if (!x_off || !a_off || !z_off) return TLPC_EINVAL;
+  int64_t c = 0;
+  for (int64_t k = 0; k < K; ++k) {
+    table[k] = x_off[k];
+    table[K + k] = a_off[k];
+    table[2 * K + k] = z_off[k];
+    table[3 * K + k] = lengths[k];
+    table[4 * K + k] = c;
+    const int64_t P = (lengths[k] + L - 1) / L;
+    for (int64_t j = 0; j < P; ++j) table[5 * K + 1 + c + j] = k;
+    c += P;
+  }
+  table[5 * K] = c;
+  return TLPC_OK;
+}
+
+TLPC_EXPORT size_t tlpc_lpc_ragged_workspace(int dtype, int64_t G, int32_t N) {
+  const int np = compiled_order(N);
+  if (wide_size(dtype) == 0 || np == 0 || G < 0) return 0;
+  return ragged_workspace(wide_size(dtype), G, np);
+}
+
+TLPC_EXPORT int tlpc_lpc_ragged_forward(int dtype, const void* x, const void* A, const void* zi,
+                                        void* y, const int64_t* table, int64_t K, int64_t G,
+                                        int64_t max_chunks, int32_t N, int32_t chunk_len,
+                                        void* workspace, size_t workspace_bytes,
+                                        void* hip_stream) {
+  return run_ragged_dtype(dtype, x, A, zi, y, nullptr, nullptr, nullptr, table, K, G, max_chunks,
+                          N, 0, chunk_len, workspace, workspace_bytes, hip_stream, false);
+}
+
+TLPC_EXPORT int tlpc_lpc_ragged_backward(int dtype, const void* g, const void* A, const void* zi,
+                                         const void* y, void* dx, void* dA, void* dzi,
+                                         const int64_t* table, int64_t K, int64_t G,
+                                         int64_t max_chunks, int32_t N, int32_t flags,
+                                         int32_t chunk_len, void* workspace,
+                                         size_t workspace_bytes, void* hip_stream) {
+  return run_ragged_dtype(dtype, g, A, zi, const_cast<void*>(y), dx, dA, dzi, table, K, G,
+                          max_chunks, N, flags, chunk_len, workspace, workspace_bytes, hip_stream,
+                          true);
 }
 
 TLPC_EXPORT size_t tlpc_scan_workspace(int dtype, int64_t B, int64_t T, int32_t flags,

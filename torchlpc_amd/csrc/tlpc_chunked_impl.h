@@ -41,14 +41,60 @@ struct ChunkCfg {
 };
 
 // ----------------------------------------------------------------------------
-// Staging helper: copy the A rows and the per-step input of G chunks for the
+// Summary work items. Item i of a uniform batch is chunk (i / (P-1), c) with
+// c = i % (P-1) (+1 for the adjoint); in a ragged batch item i is chunk id i,
+// and the chunks without a summary (the last of a signal forward, the first
+// adjoint) are idle. Per item the workgroup keeps, in LDS, where the chunk's A
+// rows and inputs start and how many of its samples exist.
+template <typename S, int NP>
+struct ItemTab {
+  int64_t a0[ChunkCfg<S, NP>::G];   // offset of A at the chunk's first sample
+  int64_t at[ChunkCfg<S, NP>::G];   // A stride over time
+  int64_t x0[ChunkCfg<S, NP>::G];   // offset of the chunk's first input sample
+  int64_t rem[ChunkCfg<S, NP>::G];  // samples of the signal from the chunk start (<= 0: idle)
+};
+
+template <typename S, int NP>
+__device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP>& tab, int64_t first,
+                                           bool fwd) {
+  using C = ChunkCfg<S, NP>;
+  const int gg = threadIdx.x;
+  if (gg >= C::G) return;
+  const int64_t item = first + gg;
+  int64_t k = 0, c = 0;
+  bool ok;
+  if (a.rg) {
+    ok = item < a.G;
+    if (ok) {
+      k = chunk_signal(a, item);
+      c = item - sig_span(a, k).cbase;
+    }
+  } else {
+    const int64_t cps = a.P - 1;
+    ok = item < a.B * cps;
+    if (ok) {
+      k = item / cps;
+      c = item - k * cps + (fwd ? 0 : 1);
+    }
+  }
+  tab.rem[gg] = -1;
+  if (!ok) return;
+  const Span sp = sig_span(a, k);
+  if (fwd ? (c >= sp.P - 1) : (c < 1)) return;
+  const int64_t t = c * a.L;
+  tab.a0[gg] = sp.ao + t * sp.at;
+  tab.at[gg] = sp.at;
+  tab.x0[gg] = sp.xo + t;
+  tab.rem[gg] = sp.T - t;
+}
+
+// Staging helper: copy the A rows and the per-step input of the G items for the
 // time block [t0, t0 + TB) into LDS; rows beyond the signal end and lags >= n
 // are zero.
 template <typename S, int NP>
 __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide<S>::type* sA,
                                             typename Wide<S>::type* sX, const S* inp,
-                                            int64_t first, int64_t nchunks, int64_t cpsig,
-                                            int64_t c_base, int t0) {
+                                            const ItemTab<S, NP>& tab, int t0) {
   using C = ChunkCfg<S, NP>;
   using W = typename Wide<S>::type;
   const int nthr = blockDim.x;
@@ -57,27 +103,15 @@ __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide
     const int r = idx - gg * (C::TB * NP);
     const int tb = r / NP;
     const int i = r - tb * NP;
-    const int64_t cid = first + gg;
     W v = Num<W>::zero();
-    if (cid < nchunks && i < a.n) {
-      const int64_t b = cid / cpsig;
-      const int64_t c = cid - b * cpsig + c_base;
-      const int64_t t = c * a.L + t0 + tb;
-      if (t < a.T) v = convert<W>(coef(a, a.A[b * a.a_bstride + t * a.a_tstride + i]));
-    }
+    if (t0 + tb < tab.rem[gg] && i < a.n) v = convert<W>(coef(a, a.A[tab.a0[gg] + (t0 + tb) * tab.at[gg] + i]));
     sA[gg * C::ASTRIDE + tb * NP + i] = v;
   }
   for (int idx = threadIdx.x; idx < C::G * C::TB; idx += nthr) {
     const int gg = idx / C::TB;
     const int tb = idx - gg * C::TB;
-    const int64_t cid = first + gg;
     W v = Num<W>::zero();
-    if (cid < nchunks) {
-      const int64_t b = cid / cpsig;
-      const int64_t c = cid - b * cpsig + c_base;
-      const int64_t t = c * a.L + t0 + tb;
-      if (t < a.T) v = convert<W>(inp[b * a.T + t]);
-    }
+    if (t0 + tb < tab.rem[gg]) v = convert<W>(inp[tab.x0[gg] + t0 + tb]);
     sX[gg * C::TB + tb] = v;
   }
 }
@@ -90,12 +124,13 @@ __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   __shared__ W sA[C::G * C::ASTRIDE];
   __shared__ W sX[C::G * C::TB];
-  const int64_t cpsig = a.P - 1;
-  const int64_t nchunks = a.B * cpsig;
+  __shared__ ItemTab<S, NP> tab;
   const int64_t first = (int64_t)blockIdx.x * C::G;
+  fill_items<S, NP>(a, tab, first, true);
+  __syncthreads();
   const int g = threadIdx.x / (NP + 1);
   const int k = threadIdx.x - g * (NP + 1);  // k < NP: basis e_k; k == NP: zero state
-  const bool active = g < C::G && first + g < nchunks;
+  const bool active = g < C::G && tab.rem[g] > 0;
 
   W ring[NP];
 #pragma unroll
@@ -107,7 +142,7 @@ __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
   const W xmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
 
   for (int t0 = 0; t0 < a.L; t0 += C::TB) {
-    stage_block<S, NP>(a, sA, sX, a.x, first, nchunks, cpsig, 0, t0);
+    stage_block<S, NP>(a, sA, sX, a.x, tab, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
@@ -142,12 +177,13 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   __shared__ W sA[C::G * C::ASTRIDE];
   __shared__ W sX[C::G * C::TB];
-  const int64_t cpsig = a.P - 1;
-  const int64_t nchunks = a.B * cpsig;
+  __shared__ ItemTab<S, NP> tab;
   const int64_t first = (int64_t)blockIdx.x * C::G;
+  fill_items<S, NP>(a, tab, first, false);
+  __syncthreads();
   const int g = threadIdx.x / (NP + 1);
   const int k = threadIdx.x - g * (NP + 1);
-  const bool active = g < C::G && first + g < nchunks;
+  const bool active = g < C::G && tab.rem[g] > 0;
 
   W ring[NP];
 #pragma unroll
@@ -159,7 +195,7 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   const W gmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
 
   for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
-    stage_block<S, NP>(a, sA, sX, a.x, first, nchunks, cpsig, 1, t0);
+    stage_block<S, NP>(a, sA, sX, a.x, tab, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
@@ -199,19 +235,17 @@ template <typename S, int NP, bool FWD>
 __global__ __launch_bounds__(64) void carry_serial(ChunkArgs<S> a) {
   const int64_t b = blockIdx.x;
   const int r = threadIdx.x;
-  const int64_t P = a.P;
-  const int64_t cpsig = P - 1;
+  const Span sp = sig_span(a, b);
+  const int64_t P = sp.P;
   using W = typename Wide<S>::type;
   W s = Num<W>::zero();
-  if (FWD && r < a.n && a.zi) s = convert<W>(a.zi[b * a.n + r]);
-  W* carry = a.carry + b * P * NP;
-  const W* summ = a.summ + b * cpsig * ChunkCfg<S, NP>::SUMM;
+  if (FWD && r < a.n && a.zi) s = convert<W>(a.zi[sp.zo + r]);
+  W* carry = a.carry + sp.cbase * NP;
   for (int64_t step = 0; step < P; ++step) {
     const int64_t c = FWD ? step : P - 1 - step;
     if (r < NP) carry[c * NP + r] = s;
     if (step == P - 1) break;
-    // summary index: forward chunk c is at c; adjoint chunk c (>=1) is at c-1
-    const W* sm = summ + (FWD ? c : c - 1) * ChunkCfg<S, NP>::SUMM;
+    const W* sm = a.summ + summ_slot(a, b, sp, c, FWD) * ChunkCfg<S, NP>::SUMM;
     W acc = (r < NP) ? sm[NP * NP + r] : Num<W>::zero();
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -280,19 +314,22 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
   __shared__ Affine<W, NP> sh[kBlockedThreads];
   const int64_t b = blockIdx.x;
   const int tid = threadIdx.x;
-  const int64_t P = a.P;
+  const Span sp = sig_span(a, b);
+  const int64_t P = sp.P;
   const int64_t links = P - 1;  // number of affine maps in the chain
   const int64_t per = (links + kBlockedThreads - 1) / kBlockedThreads;
   const int64_t lo = tid * per;
   const int64_t hi = (lo + per < links) ? lo + per : links;
-  const W* summ = a.summ + b * links * ChunkCfg<S, NP>::SUMM;
   // link l (0-based, in processing order) maps carry of step l to carry of step l+1;
-  // forward: chunk c = l at summary l; adjoint: chunk c = P-1-l at summary P-2-l.
+  // forward: chunk c = l; adjoint: chunk c = P-1-l.
+  auto link = [&](int64_t l) -> const W* {
+    return a.summ + summ_slot(a, b, sp, FWD ? l : P - 1 - l, FWD) * ChunkCfg<S, NP>::SUMM;
+  };
   Affine<W, NP> f;
   affine_identity(f);
   for (int64_t l = lo; l < hi; ++l) {
     Affine<W, NP> m;
-    load_affine(m, summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM);
+    load_affine(m, link(l));
     f = affine_compose(m, f);
   }
   sh[tid] = f;
@@ -313,7 +350,7 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
   W s[NP];
 #pragma unroll
   for (int r = 0; r < NP; ++r)
-    s[r] = (FWD && r < a.n && a.zi) ? convert<W>(a.zi[b * a.n + r]) : Num<W>::zero();
+    s[r] = (FWD && r < a.n && a.zi) ? convert<W>(a.zi[sp.zo + r]) : Num<W>::zero();
   if (tid > 0) {
     const Affine<W, NP>& pre = sh[tid - 1];
     W t[NP];
@@ -327,12 +364,12 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
 #pragma unroll
     for (int r = 0; r < NP; ++r) s[r] = t[r];
   }
-  W* carry = a.carry + b * P * NP;
+  W* carry = a.carry + sp.cbase * NP;
   for (int64_t l = lo; l < hi; ++l) {
     const int64_t c = FWD ? l : P - 1 - l;
 #pragma unroll
     for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
-    const W* sm = summ + (FWD ? l : links - 1 - l) * ChunkCfg<S, NP>::SUMM;
+    const W* sm = link(l);
     W t[NP];
 #pragma unroll
     for (int r = 0; r < NP; ++r) {
@@ -379,26 +416,27 @@ template <typename S, int NP>
 __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cid >= a.B * a.P) return;
-  const int64_t b = cid / a.P;
-  const int64_t c = cid - b * a.P;
+  if (cid >= (a.rg ? a.G : a.B * a.P)) return;
+  const int64_t b = chunk_signal(a, cid);
+  const Span sp = sig_span(a, b);
+  const int64_t c = cid - sp.cbase;
   const int64_t t0 = c * a.L;
-  const int64_t rem = a.T - t0;
+  const int64_t rem = sp.T - t0;
   const int len = rem < a.L ? (int)rem : a.L;
   W ring[NP];
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
-  const S* xb = a.x + b * a.T;
-  const S* Ab = a.A + b * a.a_bstride;
-  S* yb = a.y + b * a.T;
+  const S* xb = a.x + sp.xo;
+  const S* Ab = a.A + sp.ao;
+  S* yb = a.y + sp.xo;
   for (int jb = 0; jb < len; jb += NP) {
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
       if (jb + j < len) {
         const int64_t t = t0 + jb + j;
         S row[NP];
-        load_row<S, NP>(row, a, Ab + t * a.a_tstride);
+        load_row<S, NP>(row, a, Ab + t * sp.at);
         W acc = convert<W>(xb[t]);
 #pragma unroll
         for (int i = NP; i >= 1; --i)
@@ -417,24 +455,26 @@ template <typename S, int NP>
 __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cid >= a.B * a.P) return;
-  const int64_t b = cid / a.P;
-  const int64_t c = cid - b * a.P;
+  if (cid >= (a.rg ? a.G : a.B * a.P)) return;
+  const int64_t b = chunk_signal(a, cid);
+  const Span sp = sig_span(a, b);
+  const int64_t c = cid - sp.cbase;
   const int64_t t0 = c * a.L;
+  const int64_t T = sp.T;
   W ring[NP];  // pending sums, slot u mod NP for position u
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
-  const S* gb = a.x + b * a.T;  // upstream gradient
-  const S* Ab = a.A + b * a.a_bstride;
-  const S* yb = a.y + b * a.T;
-  const S* zib = a.zi ? a.zi + b * a.n : nullptr;
-  S* dxb = a.dx ? a.dx + b * a.T : nullptr;
-  S* dAb = a.dA ? a.dA + b * a.T * a.n : nullptr;
+  const S* gb = a.x + sp.xo;  // upstream gradient
+  const S* Ab = a.A + sp.ao;
+  const S* yb = a.y + sp.xo;
+  const S* zib = a.zi ? a.zi + sp.zo : nullptr;
+  S* dxb = a.dx ? a.dx + sp.xo : nullptr;
+  S* dAb = a.dA ? a.dA + sp.dao : nullptr;
   const bool need_da = a.dA != nullptr || a.dapart != nullptr;
   // y ring: slot u mod NP holds y_u (zi for u < 0), for the window t-1 .. t-NP.
   auto yval = [&](int64_t u) -> S {
-    if (u >= 0) return (u < a.T) ? yb[u] : Num<S>::zero();
+    if (u >= 0) return (u < T) ? yb[u] : Num<S>::zero();
     const int64_t k = -u - 1;
     return (zib && k < a.n) ? zib[k] : Num<S>::zero();
   };
@@ -454,9 +494,9 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
     for (int jj = 0; jj < NP; ++jj) {
       const int j = NP - 1 - jj;  // t mod NP
       const int64_t t = tstart - jb - jj;
-      if (t < a.T) {
+      if (t < T) {
         S row[NP];
-        load_row<S, NP>(row, a, Ab + t * a.a_tstride);
+        load_row<S, NP>(row, a, Ab + t * sp.at);
         const W lam = convert<W>(gb[t]) + ring[j];
         ring[j] = Num<W>::zero();
 #pragma unroll
@@ -491,7 +531,7 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
     // pending sums for positions -1-r sit in slots NP-1-r
 #pragma unroll
     for (int r = 0; r < NP; ++r)
-      if (r < a.n) a.dzi[b * a.n + r] = convert<S>(ring[NP - 1 - r]);
+      if (r < a.n) a.dzi[sp.zo + r] = convert<S>(ring[NP - 1 - r]);
   }
 }
 
@@ -511,12 +551,21 @@ __global__ __launch_bounds__(256) void ti_da_reduce(ChunkArgs<S> a) {
 
 // ----------------------------------------------------------------------------
 // Host-side launch of the three phases.
+// Summary work items (see fill_items) and chunk count of a batch.
+template <typename S>
+int64_t summary_items(const ChunkArgs<S>& a) {
+  return a.rg ? a.G : a.B * (a.P - 1);
+}
+template <typename S>
+int64_t chunk_count(const ChunkArgs<S>& a) {
+  return a.rg ? a.G : a.B * a.P;
+}
+
 template <typename S, int NP>
 int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
   if (a.P > 1) {
-    const int64_t nsum = a.B * (a.P - 1);
-    const int64_t grid = (nsum + C::G - 1) / C::G;
+    const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
     hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
   }
   bool blocked = false;
@@ -528,7 +577,7 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   }
   if (!blocked)
     hipLaunchKernelGGL((carry_serial<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
-  const int64_t nch = a.B * a.P;
+  const int64_t nch = chunk_count(a);
   hipLaunchKernelGGL((fwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
@@ -537,8 +586,7 @@ template <typename S, int NP>
 int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
   if (a.P > 1) {
-    const int64_t nsum = a.B * (a.P - 1);
-    const int64_t grid = (nsum + C::G - 1) / C::G;
+    const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
     hipLaunchKernelGGL((bwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
   }
   bool blocked = false;
@@ -550,7 +598,7 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   }
   if (!blocked)
     hipLaunchKernelGGL((carry_serial<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
-  const int64_t nch = a.B * a.P;
+  const int64_t nch = chunk_count(a);
   hipLaunchKernelGGL((bwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
   if (a.dapart) {
     const int64_t nd = a.B * a.n;

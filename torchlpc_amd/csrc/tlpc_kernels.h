@@ -30,7 +30,72 @@ struct ChunkArgs {
   typename Wide<S>::type* summ;   // chunk summaries [B, P-1, NP*NP+NP]
   typename Wide<S>::type* carry;  // chunk carries [B, P, NP]
   typename Wide<S>::type* wsum;   // order-16 adjoint: zero-state responses [B, P-1, NP]
+  // Ragged batch (null: a uniform [B, T] batch). B signals of one order n packed
+  // back to back; rg is a device table of int64:
+  //   rg[0..B)          sample offset of each signal in x / y / g / dx
+  //   rg[B..2B)         element offset of A[0, 0] (and dA) of each signal ([T_k, n] rows)
+  //   rg[2B..3B)        element offset of zi / dzi of each signal
+  //   rg[3B..4B)        length T_k
+  //   rg[4B..5B]        first chunk id of each signal (rg[5B] = G, the chunk count)
+  //   rg[5B+1..5B+1+G)  signal of each chunk
+  // Summaries and carries are indexed by chunk id; P holds the largest P_k.
+  const int64_t* rg;
+  int64_t G;
 };
+
+// One signal's view of the batch (uniform or ragged).
+struct Span {
+  int64_t T;      // length
+  int64_t P;      // chunks
+  int64_t xo;     // offset of sample 0 in x / y / g / dx
+  int64_t ao;     // offset of A[0, 0]
+  int64_t at;     // element stride of A over time
+  int64_t dao;    // offset of dA[0, 0] (per-sample gradient)
+  int64_t zo;     // offset of zi / dzi
+  int64_t cbase;  // chunk id of chunk 0 (carry index = cbase + c)
+};
+
+template <typename S>
+__device__ __forceinline__ Span sig_span(const ChunkArgs<S>& a, int64_t k) {
+  Span s;
+  if (a.rg) {
+    const int64_t K = a.B;
+    s.xo = a.rg[k];
+    s.ao = a.rg[K + k];
+    s.zo = a.rg[2 * K + k];
+    s.T = a.rg[3 * K + k];
+    s.cbase = a.rg[4 * K + k];
+    s.P = a.rg[4 * K + k + 1] - s.cbase;
+    s.at = a.n;
+    s.dao = s.ao;
+  } else {
+    s.xo = k * a.T;
+    s.ao = k * a.a_bstride;
+    s.zo = k * a.n;
+    s.T = a.T;
+    s.cbase = k * a.P;
+    s.P = a.P;
+    s.at = a.a_tstride;
+    s.dao = k * a.T * a.n;
+  }
+  return s;
+}
+
+// Signal of chunk id `cid` (carry index).
+template <typename S>
+__device__ __forceinline__ int64_t chunk_signal(const ChunkArgs<S>& a, int64_t cid) {
+  return a.rg ? a.rg[5 * a.B + 1 + cid] : cid / a.P;
+}
+
+// Summary slot of chunk c of a signal. Uniform batches store the forward
+// summaries of chunks 0..P-2 and the adjoint ones of chunks 1..P-1 densely;
+// ragged batches index them by chunk id.
+template <typename S>
+__device__ __forceinline__ int64_t summ_slot(const ChunkArgs<S>& a, int64_t k, const Span& s,
+                                             int64_t c, bool fwd) {
+  if (a.rg) return s.cbase + c;
+  return k * (a.P - 1) + (fwd ? c : c - 1);
+}
 
 template <typename S, int NP>
 int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st);

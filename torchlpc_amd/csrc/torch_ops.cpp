@@ -220,6 +220,99 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> scan_backward_hip(
   return {dx, dw, dinit};
 }
 
+// ---------------------------------------------------------------------------
+// Ragged batches of one order (offsets and lengths in a device table built by
+// the host plan below).
+std::tuple<at::Tensor, int64_t, int64_t, int64_t> lpc_ragged_plan_cpu(
+    const at::Tensor& x_off, const at::Tensor& a_off, const at::Tensor& z_off,
+    const at::Tensor& lengths, int64_t N, at::ScalarType dtype, int64_t chunk_len) {
+  for (const auto* t : {&x_off, &a_off, &z_off, &lengths})
+    TORCH_CHECK(t->device().is_cpu() && t->scalar_type() == at::kLong && t->dim() == 1 &&
+                    t->numel() == lengths.numel(),
+                "torchlpc_amd::lpc_ragged_plan: offsets and lengths must be 1-D int64 CPU "
+                "tensors of one length");
+  const auto xo = x_off.contiguous(), ao = a_off.contiguous(), zo = z_off.contiguous(),
+             ln = lengths.contiguous();
+  const int64_t K = ln.numel();
+  const int dt = dtype_code(at::empty({0}, at::TensorOptions().dtype(dtype)));
+  int64_t tlen = 0, G = 0, maxP = 0;
+  int32_t L = 0;
+  check_rc(tlpc_lpc_ragged_plan(K, nullptr, nullptr, nullptr, ln.data_ptr<int64_t>(), (int32_t)N,
+                                dt, (int32_t)chunk_len, nullptr, &tlen, &G, &maxP, &L),
+           "ragged plan");
+  auto table = at::empty({tlen}, at::TensorOptions().dtype(at::kLong));
+  check_rc(tlpc_lpc_ragged_plan(K, xo.data_ptr<int64_t>(), ao.data_ptr<int64_t>(),
+                                zo.data_ptr<int64_t>(), ln.data_ptr<int64_t>(), (int32_t)N, dt,
+                                (int32_t)chunk_len, table.data_ptr<int64_t>(), &tlen, &G, &maxP,
+                                &L),
+           "ragged plan");
+  return {table, G, maxP, (int64_t)L};
+}
+
+void check_ragged(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
+                  const at::Tensor& table, const char* op) {
+  TORCH_CHECK(x.is_cuda(), "torchlpc_amd::", op,
+              ": tensors must be on a HIP (MI355X) device; CPU execution is not provided");
+  TORCH_CHECK(x.dim() == 1 && A.dim() == 1 && zi.dim() == 1, "torchlpc_amd::", op,
+              ": packed x, A and zi must be 1-D");
+  TORCH_CHECK(x.is_contiguous() && A.is_contiguous() && zi.is_contiguous(), "torchlpc_amd::", op,
+              ": packed tensors must be contiguous");
+  TORCH_CHECK(A.scalar_type() == x.scalar_type() && zi.scalar_type() == x.scalar_type(),
+              "torchlpc_amd::", op, ": x, A and zi must share a dtype");
+  TORCH_CHECK(table.device() == x.device() && A.device() == x.device() && zi.device() == x.device(),
+              "torchlpc_amd::", op, ": tensors must be on one device");
+  TORCH_CHECK(table.scalar_type() == at::kLong && table.is_contiguous(), "torchlpc_amd::", op,
+              ": the plan table must be a contiguous int64 device tensor");
+}
+
+void lpc_ragged_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
+                    const at::Tensor& table, at::Tensor& y, int64_t K, int64_t G, int64_t maxP,
+                    int64_t N, int64_t chunk_len) {
+  check_ragged(x, A, zi, table, "lpc_ragged_");
+  TORCH_CHECK(y.sizes() == x.sizes() && y.scalar_type() == x.scalar_type() && y.is_contiguous(),
+              "torchlpc_amd::lpc_ragged_: y must be a contiguous tensor like x");
+  TORCH_CHECK(table.numel() == 5 * K + 1 + G, "torchlpc_amd::lpc_ragged_: table size mismatch");
+  c10::DeviceGuard guard(x.device());
+  const int dt = dtype_code(x);
+  const size_t wsb = tlpc_lpc_ragged_workspace(dt, G, (int32_t)N);
+  auto ws = workspace(x, wsb);
+  check_rc(tlpc_lpc_ragged_forward(dt, x.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(),
+                                   table.data_ptr<int64_t>(), K, G, maxP, (int32_t)N,
+                                   (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr, wsb,
+                                   stream_of(x)),
+           "ragged forward");
+}
+
+void lpc_ragged_backward_hip(const at::Tensor& g, const at::Tensor& A, const at::Tensor& zi,
+                             const at::Tensor& y, const at::Tensor& table, at::Tensor& dx,
+                             at::Tensor& dA, at::Tensor& dzi, int64_t K, int64_t G, int64_t maxP,
+                             int64_t N, int64_t chunk_len, bool need_x, bool need_a,
+                             bool need_zi) {
+  check_ragged(g, A, zi, table, "lpc_ragged_backward_");
+  TORCH_CHECK(y.sizes() == g.sizes() && y.is_contiguous(),
+              "torchlpc_amd::lpc_ragged_backward_: y must be a contiguous tensor like g");
+  TORCH_CHECK(table.numel() == 5 * K + 1 + G,
+              "torchlpc_amd::lpc_ragged_backward_: table size mismatch");
+  if (need_x) TORCH_CHECK(dx.sizes() == g.sizes() && dx.is_contiguous(), "lpc_ragged_backward_: dx");
+  if (need_a) TORCH_CHECK(dA.sizes() == A.sizes() && dA.is_contiguous(), "lpc_ragged_backward_: dA");
+  if (need_zi) TORCH_CHECK(dzi.sizes() == zi.sizes() && dzi.is_contiguous(), "lpc_ragged_backward_: dzi");
+  if (!(need_x || need_a || need_zi)) return;
+  c10::DeviceGuard guard(g.device());
+  const int dt = dtype_code(g);
+  const int flags = (need_x ? TLPC_NEED_DX : 0) | (need_a ? TLPC_NEED_DA : 0) |
+                    (need_zi ? TLPC_NEED_DZI : 0);
+  const size_t wsb = tlpc_lpc_ragged_workspace(dt, G, (int32_t)N);
+  auto ws = workspace(g, wsb);
+  check_rc(tlpc_lpc_ragged_backward(dt, g.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(),
+                                    need_x ? dx.data_ptr() : nullptr,
+                                    need_a ? dA.data_ptr() : nullptr,
+                                    need_zi ? dzi.data_ptr() : nullptr,
+                                    table.data_ptr<int64_t>(), K, G, maxP, (int32_t)N, flags,
+                                    (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr, wsb,
+                                    stream_of(g)),
+           "ragged backward");
+}
+
 [[noreturn]] void cpu_unsupported(const char* op) {
   TORCH_CHECK(false, "torchlpc_amd::", op,
               ": CPU tensors are not supported; move x, A and zi to the MI355X device (.cuda())");
@@ -237,6 +330,16 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_cpu(
     const at::Tensor&, const at::Tensor&, const at::Tensor&, const at::Tensor&, bool, bool, bool,
     int64_t, int64_t, const c10::optional<at::Tensor>&) {
   cpu_unsupported("lpc_backward");
+}
+void lpc_ragged_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, const at::Tensor&,
+                    at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t) {
+  cpu_unsupported("lpc_ragged_");
+}
+void lpc_ragged_backward_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&,
+                             const at::Tensor&, const at::Tensor&, at::Tensor&, at::Tensor&,
+                             at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t, bool, bool,
+                             bool) {
+  cpu_unsupported("lpc_ragged_backward_");
 }
 at::Tensor scan_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t) {
   cpu_unsupported("scan");
@@ -260,6 +363,16 @@ TORCH_LIBRARY(torchlpc_amd, m) {
       "lpc_backward(Tensor grad_y, Tensor A, Tensor zi, Tensor y, bool need_x, bool need_a, "
       "bool need_zi, int strategy=0, int chunk_len=0, Tensor? summary=None) -> "
       "(Tensor, Tensor, Tensor)");
+  m.def(
+      "lpc_ragged_plan(Tensor x_off, Tensor a_off, Tensor z_off, Tensor lengths, int N, "
+      "ScalarType dtype, int chunk_len=0) -> (Tensor, int, int, int)");
+  m.def(
+      "lpc_ragged_(Tensor x, Tensor A, Tensor zi, Tensor table, Tensor(a!) y, int K, int G, "
+      "int max_chunks, int N, int chunk_len) -> ()");
+  m.def(
+      "lpc_ragged_backward_(Tensor g, Tensor A, Tensor zi, Tensor y, Tensor table, Tensor(a!) dx, "
+      "Tensor(b!) dA, Tensor(c!) dzi, int K, int G, int max_chunks, int N, int chunk_len, "
+      "bool need_x, bool need_a, bool need_zi) -> ()");
   m.def("scan(Tensor x, Tensor w, Tensor init, int chunk_len=0) -> Tensor");
   m.def(
       "scan_backward(Tensor grad, Tensor w, Tensor init, Tensor h, bool need_x, bool need_w, "
@@ -270,6 +383,8 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CUDA, m) {
   m.impl("lpc", &lpc_forward_hip);
   m.impl("lpc_keep", &lpc_keep_hip);
   m.impl("lpc_backward", &lpc_backward_hip);
+  m.impl("lpc_ragged_", &lpc_ragged_hip);
+  m.impl("lpc_ragged_backward_", &lpc_ragged_backward_hip);
   m.impl("scan", &scan_hip);
   m.impl("scan_backward", &scan_backward_hip);
 }
@@ -278,6 +393,9 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CPU, m) {
   m.impl("lpc", &lpc_cpu);
   m.impl("lpc_keep", &lpc_keep_cpu);
   m.impl("lpc_backward", &lpc_backward_cpu);
+  m.impl("lpc_ragged_plan", &lpc_ragged_plan_cpu);
+  m.impl("lpc_ragged_", &lpc_ragged_cpu);
+  m.impl("lpc_ragged_backward_", &lpc_ragged_backward_cpu);
   m.impl("scan", &scan_cpu);
   m.impl("scan_backward", &scan_backward_cpu);
 }

@@ -1,0 +1,124 @@
+"""Ragged batches: many signals with their own lengths and orders in one call.
+
+The reference filters rectangular ``[B, T]`` batches with one order
+(``torchlpc/__init__.py:63-104``). Here K signals are packed back to back:
+
+* ``x``:  ``(sum T_k,)``,
+* ``a``:  ``(sum T_k * N_k,)``, signal k's coefficients as ``[T_k, N_k]`` rows,
+* ``zi``: ``(sum N_k,)``, signal k's initial state as in ``sample_wise_lpc``.
+
+Each output equals ``sample_wise_lpc`` on the signal alone. A :class:`RaggedPlan`
+holds the metadata (lengths, orders, offsets, and per order a device table that
+maps chunks to signals), built once on the host; the filter then runs as one
+set of chunked-scan launches per order present (summaries, carries, reruns),
+forward and fused backward, so a whole ragged step can be captured in one HIP
+graph.
+"""
+from __future__ import annotations
+
+from typing import Any, List, Optional, Sequence, Tuple
+
+import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
+
+from ._ext import ops
+
+
+class RaggedPlan:
+    """Host-side layout of a ragged batch and its per-order device tables."""
+
+    def __init__(
+        self,
+        lengths: Sequence[int],
+        orders: Sequence[int],
+        device: torch.device | str,
+        dtype: torch.dtype = torch.float32,
+        chunk_len: int = 0,
+    ) -> None:
+        lengths_t = torch.as_tensor(list(lengths), dtype=torch.int64)
+        orders_t = torch.as_tensor(list(orders), dtype=torch.int64)
+        assert lengths_t.dim() == 1 and lengths_t.shape == orders_t.shape
+        assert bool((lengths_t >= 1).all()) and bool((orders_t >= 1).all())
+        self.lengths = lengths_t
+        self.orders = orders_t
+        self.device = torch.device(device)
+        self.dtype = dtype
+        zero = torch.zeros(1, dtype=torch.int64)
+        self.x_off = torch.cat([zero, lengths_t.cumsum(0)])
+        self.a_off = torch.cat([zero, (lengths_t * orders_t).cumsum(0)])
+        self.z_off = torch.cat([zero, orders_t.cumsum(0)])
+        self.total_samples = int(self.x_off[-1])
+        self.total_coeffs = int(self.a_off[-1])
+        self.total_state = int(self.z_off[-1])
+        # per order: (N, device table, K, G, max chunks, chunk length)
+        self.groups: List[Tuple[int, torch.Tensor, int, int, int, int]] = []
+        for n in sorted(set(orders_t.tolist())):
+            assert n <= 32, "ragged batches support orders up to 32"
+            idx = (orders_t == n).nonzero().flatten()
+            table, G, maxp, L = ops().lpc_ragged_plan(
+                self.x_off[idx].contiguous(),
+                self.a_off[idx].contiguous(),
+                self.z_off[idx].contiguous(),
+                lengths_t[idx].contiguous(),
+                n,
+                dtype,
+                chunk_len,
+            )
+            self.groups.append((n, table.to(self.device), int(idx.numel()), G, maxp, L))
+
+    def split(self, packed: torch.Tensor, kind: str = "x") -> List[torch.Tensor]:
+        """Views of the K signals of a packed tensor (kind "x", "a" or "zi")."""
+        off = {"x": self.x_off, "a": self.a_off, "zi": self.z_off}[kind]
+        out = []
+        for k in range(self.lengths.numel()):
+            v = packed[int(off[k]) : int(off[k + 1])]
+            if kind == "a":
+                v = v.view(int(self.lengths[k]), int(self.orders[k]))
+            out.append(v)
+        return out
+
+
+class RaggedLPC(Function):
+    """Packed ``y`` of the ragged batch described by a :class:`RaggedPlan`."""
+
+    @staticmethod
+    def forward(x: torch.Tensor, a: torch.Tensor, zi: torch.Tensor, plan: RaggedPlan) -> torch.Tensor:
+        y = torch.empty_like(x)
+        for n, table, K, G, maxp, L in plan.groups:
+            ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L)
+        return y
+
+    @staticmethod
+    def setup_context(ctx: Any, inputs: Tuple[Any, ...], output: Any) -> None:
+        _, a, zi, plan = inputs
+        ctx.save_for_backward(a, zi, output)
+        ctx.plan = plan
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx: Any, grad_y: torch.Tensor):
+        a, zi, y = ctx.saved_tensors
+        need_x, need_a, need_zi = ctx.needs_input_grad[:3]
+        g = grad_y.contiguous()
+        dx = torch.empty_like(g) if need_x else g.new_empty(0)
+        da = torch.empty_like(a) if need_a else a.new_empty(0)
+        dzi = torch.empty_like(zi) if need_zi else zi.new_empty(0)
+        if need_x or need_a or need_zi:
+            for n, table, K, G, maxp, L in ctx.plan.groups:
+                ops().lpc_ragged_backward_(
+                    g, a, zi, y, table, dx, da, dzi, K, G, maxp, n, L, need_x, need_a, need_zi
+                )
+        return (dx if need_x else None, da if need_a else None, dzi if need_zi else None, None)
+
+
+def ragged_lpc(
+    x: torch.Tensor, a: torch.Tensor, plan: RaggedPlan, zi: Optional[torch.Tensor] = None
+) -> torch.Tensor:
+    """Packed outputs of the ragged all-pole filter (see the module docstring)."""
+    assert x.dim() == 1 and x.numel() == plan.total_samples
+    assert a.dim() == 1 and a.numel() == plan.total_coeffs
+    if zi is None:
+        zi = a.new_zeros(plan.total_state)
+    assert zi.dim() == 1 and zi.numel() == plan.total_state
+    return RaggedLPC.apply(x.contiguous(), a.contiguous(), zi.contiguous(), plan)
