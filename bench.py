@@ -132,6 +132,170 @@ def cpu_baseline(T, N, sample_signals=64, min_seconds=10.0):
     }
 
 
+def _step_up(k):
+    """Direct-form coefficients a[..., N] from reflection coefficients k[..., N]."""
+    import torch
+
+    a = torch.zeros_like(k)
+    N = k.shape[-1]
+    for m in range(1, N + 1):
+        km = k[..., m - 1 : m]
+        prev = a[..., : m - 1].clone()
+        a[..., : m - 1] = prev + km * prev.flip(-1)
+        a[..., m - 1] = k[..., m - 1]
+    return a
+
+
+def _interp(anchors, T, hop):
+    """Linear interpolation of anchor values [..., n_anchor, N] every `hop` samples."""
+    import torch
+
+    n_anchor = anchors.shape[-2]
+    pos = torch.arange(T, device=anchors.device, dtype=anchors.dtype) / hop
+    lo = pos.floor().long().clamp(max=n_anchor - 1)
+    hi = (lo + 1).clamp(max=n_anchor - 1)
+    frac = (pos - lo)[:, None]
+    return anchors[..., lo, :] * (1 - frac) + anchors[..., hi, :] * frac
+
+
+def workload_inputs(name, device, seed):
+    """Synthetic inputs of the secondary workloads (float64, generated on the GPU)."""
+    import math
+
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    f64 = dict(device=device, dtype=torch.float64)
+    if name == "cfg1":  # B=16, T=8192, N=4: two resonators per signal, 32 anchor frames
+        B, T = 16, 8192
+        r = torch.rand(B, 32, 2, generator=g, **f64) * 0.45 + 0.5
+        th = (torch.rand(B, 32, 2, generator=g, **f64) * 0.9 + 0.05) * math.pi
+        rt, tt = _interp(r, T, (T - 1) / 31), _interp(th, T, (T - 1) / 31)
+        c1, c2 = -2 * rt * torch.cos(tt), rt * rt  # each section 1 + c1 z^-1 + c2 z^-2
+        A = torch.stack(
+            [
+                c1[..., 0] + c1[..., 1],
+                c2[..., 0] + c2[..., 1] + c1[..., 0] * c1[..., 1],
+                c1[..., 0] * c2[..., 1] + c2[..., 0] * c1[..., 1],
+                c2[..., 0] * c2[..., 1],
+            ],
+            -1,
+        )
+        return torch.randn(B, T, generator=g, **f64), A, torch.zeros(B, 4, **f64), None
+    if name == "cfg3":  # B=4, T=2^20, N=2: slowly drifting resonators near the unit circle
+        B, T = 4, 1 << 20
+        r = torch.rand(B, 1, generator=g, **f64) * 0.099 + 0.9
+        thc = (torch.rand(B, 1, generator=g, **f64) * 0.6 + 0.2) * math.pi
+        phi = torch.rand(B, 1, generator=g, **f64) * 2 * math.pi
+        t = torch.arange(T, **f64)[None]
+        th = thc + 0.1 * math.pi * torch.sin(2 * math.pi * t / 65536 + phi)
+        A = torch.stack([-2 * r * torch.cos(th), (r * r).expand(B, T)], -1)
+        return (torch.randn(B, T, generator=g, **f64), A, torch.randn(B, 2, generator=g, **f64),
+                torch.randn(B, T, generator=g, **f64))
+    if name == "cfg5":  # B=1024, T=16384, N=32, time-invariant a[B, N]
+        B, T, N = 1024, 16384, 32
+        a = _step_up((torch.rand(B, N, generator=g, **f64) * 2 - 1) * 0.8)
+        return (torch.randn(B, T, generator=g, **f64), a, torch.randn(B, N, generator=g, **f64) * 0.1,
+                torch.randn(B, T, generator=g, **f64))
+    raise ValueError(name)
+
+
+def bench_workload(name, steps, warmup):
+    """fwd(+bwd) of a secondary workload; returns the JSON record."""
+    import torch
+
+    import torchlpc_amd
+
+    dev = torch.device("cuda:0")
+    ops = torch.ops.torchlpc_amd
+    stream = torch.cuda.current_stream()
+    f32 = torch.float32
+    if name == "cfg4":  # 512 ragged signals, lengths ~ U{1024..65536}, orders in {2,4,8,16,32}
+        g = torch.Generator(device="cpu").manual_seed(1004)
+        K = 512
+        lengths = torch.randint(1024, 65537, (K,), generator=g).tolist()
+        orders = [(2, 4, 8, 16, 32)[i] for i in torch.randint(0, 5, (K,), generator=g).tolist()]
+        gd = torch.Generator(device=dev).manual_seed(1004)
+        xs, As, zs = [], [], []
+        for T, N in zip(lengths, orders):
+            kmax = 0.9 if N <= 16 else 0.5  # order 32 with 0.9 grows past float32
+            kk = (torch.rand(1, (T - 1) // 256 + 2, N, generator=gd, device=dev, dtype=torch.float64) * 2 - 1) * kmax
+            As.append(_step_up(_interp(kk, T, 256))[0].reshape(-1))
+            xs.append(torch.randn(T, generator=gd, device=dev, dtype=torch.float64))
+            zs.append(torch.randn(N, generator=gd, device=dev, dtype=torch.float64) * 0.1)
+        x, A, zi = (torch.cat(v).to(f32) for v in (xs, As, zs))
+        w = torch.randn(x.numel(), generator=gd, device=dev, dtype=f32)
+        plan = torchlpc_amd.RaggedPlan(lengths, orders, dev, f32)
+        y = torch.empty_like(x)
+        dx, dA, dzi = torch.empty_like(x), torch.empty_like(A), torch.empty_like(zi)
+
+        def step():
+            for n, table, Kg, G, maxp, L in plan.groups:
+                ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L)
+            for n, table, Kg, G, maxp, L in plan.groups:
+                ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True)
+
+        S = sum(lengths)
+        alg = sum((3 * N + 5) * 4 * T for T, N in zip(lengths, orders))
+        workload = f"ragged all-pole filter fwd + fused adjoint bwd, {K} signals, T_k~U{{1024..65536}}, N_k in {{2,4,8,16,32}}, fp32, one HIP graph"
+        cfg = {"workload": workload, "signals": K, "samples": S}
+        s = torch.cuda.Stream()
+        s.wait_stream(stream)
+        with torch.cuda.stream(s):
+            step()
+        stream.wait_stream(s)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step()
+        run = graph.replay
+    else:
+        x64, A64, zi64, w64 = workload_inputs(name, dev, {"cfg1": 1001, "cfg3": 1003, "cfg5": 1005}[name])
+        x, A, zi = x64.to(f32), A64.to(f32).contiguous(), zi64.to(f32)
+        w = w64.to(f32) if w64 is not None else None
+        B, T = x.shape
+        N = A.shape[-1]
+        S = B * T
+        if name == "cfg1":
+            def run():
+                ops.lpc(x, A, zi)
+            alg = (N + 2) * 4 * S
+            cfg = {"workload": f"time-varying all-pole filter forward, B={B}, T={T}, N={N}, fp32", "B": B, "T": T, "N": N}
+        else:
+            def run():
+                y = ops.lpc(x, A, zi)
+                ops.lpc_backward(w, A, zi, y, True, True, True)
+            ti = A.dim() == 2
+            alg = (5 if ti else 3 * N + 5) * 4 * S
+            kind = "time-invariant" if ti else "time-varying"
+            cfg = {"workload": f"{kind} all-pole filter fwd + fused adjoint bwd, B={B}, T={T}, N={N}, fp32", "B": B, "T": T, "N": N}
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    gbs = alg / (ms * 1e-3) / 1e9
+    return {
+        "metric": "Msamples/s (secondary workload)",
+        "value": S / (ms * 1e-3) / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg},
+    }
+
+
 def _short(name):
     """Kernel name without namespace, return type and argument list."""
     name = name.split("(")[0].replace("void ", "").strip()
@@ -223,6 +387,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", default="headline", choices=["headline", "cfg1", "cfg3", "cfg4", "cfg5"],
+                    help="secondary workloads print their own line (the default is the headline)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one copy-like kernel of known size (for counter scaling)")
     ap.add_argument("--trace-summary", metavar="DB", help="summarise a rocprofv3 kernel-trace database and exit")
@@ -244,6 +410,11 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+
+    if args.workload != "headline":
+        log(f"[bench] workload {args.workload}")
+        print(json.dumps(bench_workload(args.workload, args.steps, args.warmup)), flush=True)
+        return
 
     import torch
 
