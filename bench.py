@@ -37,10 +37,17 @@ COPY_BYTES = 1 << 30  # the --calibrate-copy kernel reads and writes this many b
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 # kernels of one forward and one backward call at the order-16 float32 shape
+# (name prefixes: template arguments after these select the load variant)
 OP_KERNELS = {
-    "lpc (forward)": ("sum_fwd<true>", "carry16<true>", "rerun_fwd"),
-    "lpc_backward (fused adjoint)": ("sum_adj", "carry16<false>", "rerun_adj<true>"),
+    "lpc (forward)": ("sum_fwd<true", "carry16<true>", "rerun_fwd"),
+    "lpc_backward (fused adjoint)": ("sum_adj", "carry16<false>", "rerun_adj<true"),
 }
+
+
+def _op_of(avg, prefix):
+    """Total of the kernels in `avg` whose name starts with `prefix` (None if none)."""
+    hits = [v for k, v in avg.items() if k.startswith(prefix)]
+    return sum(hits) if hits else None
 
 
 def log(*a):
@@ -319,8 +326,9 @@ def trace_summary(db_path):
         if "tlpc" in name or k.startswith("neg"):
             lines.append(f"{k:40s} {n:5d} {d / 1e3:10.1f}")
     for op, ks in OP_KERNELS.items():
-        if all(k in avg for k in ks):
-            lines.append(f"op {op}: sum of kernel averages = {sum(avg[k] for k in ks):.1f} us " f"({' + '.join(ks)})")
+        parts = [_op_of(avg, k) for k in ks]
+        if all(p is not None for p in parts):
+            lines.append(f"op {op}: sum of kernel averages = {sum(parts):.1f} us ({' + '.join(ks)})")
     return "\n".join(lines)
 
 
@@ -355,11 +363,12 @@ def pmc_summary(fetch_dir, write_dir):
             continue
         t = fetch[k] * 1024 * ff + write[k] * 1024 * fw
         traffic[k] = t
-        if any(k in ks for ks in OP_KERNELS.values()):
+        if any(k.startswith(p) for ks in OP_KERNELS.values() for p in ks):
             lines.append(f"{k:40s} {fetch[k]:12.0f} {write[k]:12.0f} {t / 1e6:11.1f}")
     for op, ks in OP_KERNELS.items():
-        if all(k in traffic for k in ks):
-            lines.append(f"traffic_bytes_per_call {op}: {sum(traffic[k] for k in ks):.0f}")
+        parts = [_op_of(traffic, k) for k in ks]
+        if all(p is not None for p in parts):
+            lines.append(f"traffic_bytes_per_call {op}: {sum(parts):.0f}")
     return "\n".join(lines)
 
 

@@ -108,6 +108,34 @@ __device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], 
 }
 
 // ---------------------------------------------------------------------------
+// Per-sample scalar streams (x, g, y). With VEC (T a multiple of 4 and 16-byte
+// aligned rows) lane ql of a quad loads samples [tb+4ql, tb+4ql+4) of a 16-step
+// block as one float4, a block ahead, and each step takes its sample from the
+// owning lane by a quad broadcast: one 64-byte request per quad per 16 steps
+// instead of one per step. Without VEC every step loads its sample.
+constexpr int qbcast_ctrl(int q) { return q * 0x55; }  // quad_perm [q,q,q,q]
+
+template <int K>
+__device__ __forceinline__ float comp(const float4& v) {
+  if constexpr ((K & 3) == 0) return v.x;
+  if constexpr ((K & 3) == 1) return v.y;
+  if constexpr ((K & 3) == 2) return v.z;
+  return v.w;
+}
+// sample K (0..15) of a block held as one float4 per quad lane, in every lane
+template <int K>
+__device__ __forceinline__ float qsel(const float4& v) {
+  return qperm<qbcast_ctrl(K >> 2)>(comp<K>(v));
+}
+
+// float4 of samples [base+4ql, base+4ql+4) of a row of length T (T % 4 == 0);
+// out-of-range blocks read a valid float4 whose values the callers never use.
+__device__ __forceinline__ float4 ld4(const float* p, int base, int ql, int T) {
+  const int i = clampi(base + 4 * ql, 0, T - 4);
+  return *reinterpret_cast<const float4*>(p + i);
+}
+
+// ---------------------------------------------------------------------------
 // Forward quad run over [t0, t0+len): 16-step blocks, a PF-deep ring of rows and
 // inputs. Steps past the end of the signal run on clamped data and store nothing.
 struct FwdIo {
@@ -117,22 +145,26 @@ struct FwdIo {
   int T;
 };
 
-template <int JJ, bool STORE, typename V>
+template <int JJ, bool STORE, bool VEC, typename V>
 __device__ __forceinline__ void fwd_q16(V (&R)[4], float4 (&cq)[PF], float (&xq)[PF], float4& out,
-                                        const FwdIo& io, int tb, int ql) {
+                                        const float4& xc, const FwdIo& io, int tb, int ql) {
   if constexpr (JJ < 16) {
-    const float4 c = cq[JJ % PF];
-    const float xv = xq[JJ % PF];
+    // the step consumes its ring slot before the slot is reloaded (for step
+    // JJ + PF), so the old and new values never overlap and the slot stays in
+    // one register across iterations (no copies, no wait at the loop latch)
+    float xv;
+    if constexpr (VEC) xv = qsel<JJ>(xc);
+    else xv = xq[JJ % PF];
+    const V y = fwd_quad_step<JJ & 3>(R, cq[JJ % PF], xv, ql);
     const int tn = min(tb + JJ + PF, io.T - 1);
     cq[JJ % PF] = ld_row(io.Ab, tn, ql);
-    xq[JJ % PF] = io.xb[tn];
-    const V y = fwd_quad_step<JJ & 3>(R, c, xv, ql);
+    if constexpr (!VEC) xq[JJ % PF] = io.xb[tn];
     if constexpr (STORE) put<JJ>(out, ql, (float)y);
-    fwd_q16<JJ + 1, STORE>(R, cq, xq, out, io, tb, ql);
+    fwd_q16<JJ + 1, STORE, VEC>(R, cq, xq, out, xc, io, tb, ql);
   }
 }
 
-template <bool STORE, typename V>
+template <bool STORE, bool VEC, typename V>
 __device__ __forceinline__ void fwd_quad_run(V (&R)[4], const FwdIo& io, int t0, int len, int ql) {
   float4 cq[PF];
   float xq[PF];
@@ -140,12 +172,18 @@ __device__ __forceinline__ void fwd_quad_run(V (&R)[4], const FwdIo& io, int t0,
   for (int j = 0; j < PF; ++j) {
     const int tj = min(t0 + j, io.T - 1);
     cq[j] = opaque(ld_row(io.Ab, tj, ql));
-    xq[j] = opaque(io.xb[tj]);
+    if constexpr (!VEC) xq[j] = opaque(io.xb[tj]);
   }
+  float4 xc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if constexpr (VEC) xc = opaque(ld4(io.xb, t0, ql, io.T));
   for (int jb = 0; jb < len; jb += 16) {
+    const int tb = t0 + jb;
+    float4 xn = xc;
+    if constexpr (VEC) xn = ld4(io.xb, tb + 16, ql, io.T);  // used by the next block
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-    fwd_q16<0, STORE>(R, cq, xq, out, io, t0 + jb, ql);
-    if constexpr (STORE) store16(io.yb, t0 + jb, io.T, ql, out);
+    fwd_q16<0, STORE, VEC>(R, cq, xq, out, xc, io, tb, ql);
+    if constexpr (STORE) store16(io.yb, tb, io.T, ql, out);
+    xc = xn;
   }
 }
 
@@ -164,24 +202,40 @@ struct AdjIo {
   int T;
 };
 
-template <int JJ, bool OUT, bool DA>
+// VEC blocks of the adjoint: g of the current block, and the y samples that
+// enter the window (y_{t-17} at step t): block tb-16 for the first 15 steps of
+// block tb, block tb-32 for its last step.
+struct AdjVec {
+  float4 g, ycur, ynext;
+};
+
+template <int JJ, bool OUT, bool DA, bool VEC>
 __device__ __forceinline__ void adj_q16(double (&P)[4], double& now, float (&Y)[4],
                                         float4 (&cq)[PF], float (&gq)[PF], float (&yq)[PF],
-                                        float4& dxo, const AdjIo& io, int tb, int ql) {
+                                        float4& dxo, const AdjVec& vb, const AdjIo& io, int tb,
+                                        int ql) {
   if constexpr (JJ < 16) {
     constexpr int J = (3 - JJ) & 3;  // slot of t = tb + 15 - JJ (tb is a multiple of 16)
     const int t = tb + 15 - JJ;
-    const float4 cv = cq[JJ % PF];
-    const float gv = gq[JJ % PF];
-    const float yraw = yq[JJ % PF];
+    float gv, yraw = 0.f;
+    if constexpr (VEC) {
+      gv = qsel<15 - JJ>(vb.g);
+      if constexpr (DA) yraw = (JJ < 15) ? qsel<(14 - JJ) & 15>(vb.ycur) : qsel<15>(vb.ynext);
+    } else {
+      gv = gq[JJ % PF];
+      yraw = yq[JJ % PF];
+    }
+    const bool live = t < io.T;
+    // ring slots are consumed before they are reloaded (see fwd_q16)
+    const double lam = adj_quad_step<J>(P, now, cq[JJ % PF], gv, ql, live);
     {
       const int tc = clampi(t - PF, 0, io.T - 1);
       cq[JJ % PF] = ld_row(io.Ab, tc, ql);
-      gq[JJ % PF] = io.gb[tc];
-      if constexpr (DA) yq[JJ % PF] = *yaddr(io.yb, io.zib, t - PF - 17, io.T);
+      if constexpr (!VEC) {
+        gq[JJ % PF] = io.gb[tc];
+        if constexpr (DA) yq[JJ % PF] = *yaddr(io.yb, io.zib, t - PF - 17, io.T);
+      }
     }
-    const bool live = t < io.T;
-    const double lam = adj_quad_step<J>(P, now, cv, gv, ql, live);
     if constexpr (OUT) {
       const float lf = (float)lam;
       put<15 - JJ>(dxo, ql, lf);
@@ -196,15 +250,30 @@ __device__ __forceinline__ void adj_q16(double (&P)[4], double& now, float (&Y)[
         }
         constexpr int S = (J - 1) & 3;
         const float down = qperm<QP_DOWN>(Y[S]);
-        const float ynew = yok(io.zib, t - 17) ? yraw : 0.f;
+        // VEC blocks already hold zi (or zero) before time 0
+        const float ynew = (VEC || yok(io.zib, t - 17)) ? yraw : 0.f;
         Y[S] = (ql == 3) ? ynew : down;  // lane 3 takes y_{t-17}
       }
     }
-    adj_q16<JJ + 1, OUT, DA>(P, now, Y, cq, gq, yq, dxo, io, tb, ql);
+    adj_q16<JJ + 1, OUT, DA, VEC>(P, now, Y, cq, gq, yq, dxo, vb, io, tb, ql);
   }
 }
 
-template <bool OUT, bool DA>
+// y block [v, v+16) as one float4 per lane, for a block start v (a multiple of
+// 16) at or after -32: v >= 0 loads (clamped), v == -16 is the initial state
+// reversed (y_{-1-k} = zi[k], zrev precomputed), v == -32 is zero.
+__device__ __forceinline__ float4 yblock_fix(const float4& raw, const float4& zrev, int v) {
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4& alt = (v == -16) ? zrev : z;
+  float4 r;
+  r.x = v >= 0 ? raw.x : alt.x;
+  r.y = v >= 0 ? raw.y : alt.y;
+  r.z = v >= 0 ? raw.z : alt.z;
+  r.w = v >= 0 ? raw.w : alt.w;
+  return r;
+}
+
+template <bool OUT, bool DA, bool VEC>
 __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const AdjIo& io, int t0,
                                              int L, int ql) {
   const int tstart = t0 + L - 1;
@@ -225,21 +294,55 @@ __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const 
   for (int j = 0; j < PF; ++j) {
     const int tc = clampi(tstart - j, 0, io.T - 1);
     cq[j] = opaque(ld_row(io.Ab, tc, ql));
-    gq[j] = opaque(io.gb[tc]);
-    yq[j] = DA ? opaque(*yaddr(io.yb, io.zib, tstart - j - 17, io.T)) : 0.f;
+    if constexpr (!VEC) {
+      gq[j] = opaque(io.gb[tc]);
+      yq[j] = DA ? opaque(*yaddr(io.yb, io.zib, tstart - j - 17, io.T)) : 0.f;
+    }
   }
-  for (int tb = t0 + L - 16; tb >= t0; tb -= 16) {
+  const int tb0 = t0 + L - 16;
+  float4 gc = make_float4(0.f, 0.f, 0.f, 0.f), yc = gc, yn = gc, zrev = gc;
+  if constexpr (VEC) {
+    gc = opaque(ld4(io.gb, tb0, ql, io.T));
+    if constexpr (DA) {
+      yc = opaque(ld4(io.yb, tb0 - 16, ql, io.T));
+      yn = opaque(ld4(io.yb, tb0 - 32, ql, io.T));
+      // zi as y_{-16..-1}: lane ql, component m holds y_{-16+4ql+m} = zi[15-4ql-m]
+      if (io.zib) {
+        const float4 zv = *reinterpret_cast<const float4*>(io.zib + 4 * ql);
+        constexpr int REV = 0x1B;  // quad_perm [3,2,1,0]
+        zrev.x = qperm<REV>(zv.w);
+        zrev.y = qperm<REV>(zv.z);
+        zrev.z = qperm<REV>(zv.y);
+        zrev.w = qperm<REV>(zv.x);
+      }
+    }
+  }
+  for (int tb = tb0; tb >= t0; tb -= 16) {
+    AdjVec vb;
+    float4 gn = gc, yn2 = yn;
+    if constexpr (VEC) {
+      gn = ld4(io.gb, tb - 16, ql, io.T);  // used by the next block
+      vb.g = gc;
+      if constexpr (DA) {
+        yn2 = ld4(io.yb, tb - 48, ql, io.T);
+        vb.ycur = yblock_fix(yc, zrev, tb - 16);
+        vb.ynext = yblock_fix(yn, zrev, tb - 32);
+      }
+    }
     float4 dxo = make_float4(0.f, 0.f, 0.f, 0.f);
-    adj_q16<0, OUT, DA>(P, now, Y, cq, gq, yq, dxo, io, tb, ql);
+    adj_q16<0, OUT, DA, VEC>(P, now, Y, cq, gq, yq, dxo, vb, io, tb, ql);
     if constexpr (OUT) {
       if (io.dxb) store16(io.dxb, tb, io.T, ql, dxo);
     }
+    gc = gn;
+    yc = yn;
+    yn = yn2;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Forward summaries: chunks c = 0..P-2 (full length L).
-template <bool WITH_Z>
+template <bool WITH_Z, bool VEC>
 __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -274,7 +377,7 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
     const int c = (int)(cid - b * cps);
     double R[4] = {0.0, 0.0, 0.0, 0.0};
     const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, T};
-    fwd_quad_run<false>(R, io, c * L, L, ql);
+    fwd_quad_run<false, VEC>(R, io, c * L, L, ql);
     double* out = a.summ + cid * (NP * NP + NP) + NP * NP;
 #pragma unroll
     for (int m = 0; m < 4; ++m) out[4 * ql + m] = R[3 - m];  // z[4ql+m] = y_{L-1-4ql-m}
@@ -284,6 +387,7 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
 // Adjoint zero-state responses: chunks c = 1..P-1, one quad each (the last may
 // be partial; its steps past the end are no-ops of the zero state).
 constexpr int WSUM_THREADS = 256;
+template <bool VEC>
 __global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
   const int64_t cps = a.P - 1;
   const int64_t cid = ((int64_t)blockIdx.x * WSUM_THREADS + threadIdx.x) >> 2;
@@ -294,7 +398,7 @@ __global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
   double P[4] = {0.0, 0.0, 0.0, 0.0};
   double now = 0.0;
   const AdjIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, nullptr, nullptr, nullptr, (int)a.T};
-  adj_quad_run<false, false>(P, now, io, c * a.L, a.L, ql);
+  adj_quad_run<false, false, VEC>(P, now, io, c * a.L, a.L, ql);
   // w[0] = now (lane 0); w[4ql+1+m] = P[(2-m) & 3]
   double* out = a.wsum + cid * NP;
   if (ql == 0) out[0] = now;
@@ -307,6 +411,7 @@ __global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
 
 // ---------------------------------------------------------------------------
 // Forward rerun: one quad per chunk (all P chunks of all signals).
+template <bool VEC>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
   const int64_t cid = ((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2;
   if (cid >= a.B * a.P) return;
@@ -321,12 +426,12 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
 #pragma unroll
   for (int m = 0; m < 4; ++m) R[3 - m] = cr[4 * ql + m];  // y_{t0-1-4ql-m}, slot 3-m
   const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, a.y + b * a.T, T};
-  fwd_quad_run<true>(R, io, t0, len, ql);
+  fwd_quad_run<true, VEC>(R, io, t0, len, ql);
 }
 
 // Adjoint rerun: one quad per chunk, descending from the chunk's last step (steps
 // past the end of the last chunk are no-ops of its zero carry).
-template <bool DA>
+template <bool DA, bool VEC>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
   const int64_t cid = ((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2;
   if (cid >= a.B * a.P) return;
@@ -351,7 +456,7 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
                  a.dx ? a.dx + b * a.T : nullptr,
                  a.dA ? a.dA + b * a.T * NP : nullptr,
                  T};
-  adj_quad_run<true, DA>(P, now, io, c * L, L, ql);
+  adj_quad_run<true, DA, VEC>(P, now, io, c * L, L, ql);
   if (c == 0 && a.dzi) {
     // pending of positions -1-r: r = 0 in `now` (lane 0), r = 4ql+1+m in slot (2-m)&3
     float* dz = a.dzi + b * NP;
@@ -446,35 +551,55 @@ int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// The per-sample streams are read as float4 blocks when every row starts on a
+// 16-byte boundary: T a multiple of 4 and 16-byte aligned x / g, y and zi.
+static bool vec_ok(const ChunkArgs<float>& a) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  return a.T % 4 == 0 && al(a.x) && (!a.y || al(a.y)) && (!a.zi || al(a.zi));
+}
+
 int launch_chunk16_fwd_summary(ChunkArgs<float> a, bool with_z, hipStream_t st) {
   const int64_t n = a.B * (a.P - 1);
   const dim3 grid((unsigned)((n + 15) / 16));
-  if (n > 0 && with_z)
-    hipLaunchKernelGGL(c16::sum_fwd<true>, grid, dim3(c16::SUM_THREADS), 0, st, a);
+  const bool v = vec_ok(a);
+  if (n > 0 && with_z && v)
+    hipLaunchKernelGGL((c16::sum_fwd<true, true>), grid, dim3(c16::SUM_THREADS), 0, st, a);
+  else if (n > 0 && with_z)
+    hipLaunchKernelGGL((c16::sum_fwd<true, false>), grid, dim3(c16::SUM_THREADS), 0, st, a);
   else if (n > 0)
-    hipLaunchKernelGGL(c16::sum_fwd<false>, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((c16::sum_fwd<false, false>), grid, dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_adj_summary(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * (a.P - 1) * 4;
-  if (lanes > 0)
-    hipLaunchKernelGGL(c16::sum_adj, dim3((unsigned)((lanes + c16::WSUM_THREADS - 1) / c16::WSUM_THREADS)),
-                       dim3(c16::WSUM_THREADS), 0, st, a);
+  const dim3 grid((unsigned)((lanes + c16::WSUM_THREADS - 1) / c16::WSUM_THREADS));
+  if (lanes > 0 && vec_ok(a))
+    hipLaunchKernelGGL(c16::sum_adj<true>, grid, dim3(c16::WSUM_THREADS), 0, st, a);
+  else if (lanes > 0)
+    hipLaunchKernelGGL(c16::sum_adj<false>, grid, dim3(c16::WSUM_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * a.P * 4;
-  hipLaunchKernelGGL(c16::rerun_fwd, dim3((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS)),
-                     dim3(c16::RERUN_THREADS), 0, st, a);
+  const dim3 grid((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS));
+  if (vec_ok(a))
+    hipLaunchKernelGGL(c16::rerun_fwd<true>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else
+    hipLaunchKernelGGL(c16::rerun_fwd<false>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * a.P * 4;
   const dim3 grid((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS));
-  if (a.dA)
-    hipLaunchKernelGGL(c16::rerun_adj<true>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  const bool v = vec_ok(a);
+  if (a.dA && v)
+    hipLaunchKernelGGL((c16::rerun_adj<true, true>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else if (a.dA)
+    hipLaunchKernelGGL((c16::rerun_adj<true, false>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else if (v)
+    hipLaunchKernelGGL((c16::rerun_adj<false, true>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
   else
-    hipLaunchKernelGGL(c16::rerun_adj<false>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
+    hipLaunchKernelGGL((c16::rerun_adj<false, false>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
