@@ -411,10 +411,17 @@ __global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
 
 // ---------------------------------------------------------------------------
 // Forward rerun: one quad per chunk (all P chunks of all signals).
+// The reruns walk the chunks from the last one down: the A rows that the
+// summary kernel before them read last are then read first, while they may
+// still be in the last-level (MALL) cache.
+__device__ __forceinline__ int64_t rerun_chunk(const ChunkArgs<float>& a) {
+  return a.B * a.P - 1 - (((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2);
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
-  const int64_t cid = ((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2;
-  if (cid >= a.B * a.P) return;
+  const int64_t cid = rerun_chunk(a);
+  if (cid < 0) return;
   const int ql = threadIdx.x & 3;
   const int64_t b = cid / a.P;
   const int c = (int)(cid - b * a.P);
@@ -433,8 +440,8 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
 // past the end of the last chunk are no-ops of its zero carry).
 template <bool DA, bool VEC>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
-  const int64_t cid = ((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2;
-  if (cid >= a.B * a.P) return;
+  const int64_t cid = rerun_chunk(a);
+  if (cid < 0) return;
   const int ql = threadIdx.x & 3;
   const int64_t b = cid / a.P;
   const int c = (int)(cid - b * a.P);
@@ -470,28 +477,42 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Carry chain, one wave per signal: s_{l+1} = M_l s_l + v_l over the P-1 links.
-// Forward: link l is chunk l, M = Phi_l, v = z_l. Adjoint: link l is chunk
-// P-1-l, M = Phi_{P-1-l}^T, v = w_{P-1-l}; the first link (the last chunk) meets
-// the zero state past the end of the signal, so it needs no matrix. Link data
-// comes through LDS in batches of CB links: the loads of batch n+1 are in flight
-// while batch n is chained.
+// Carry chain, one workgroup per signal: s_{l+1} = M_l s_l + v_l over the P-1
+// links. Forward: link l is chunk l, M = Phi_l, v = z_l. Adjoint: link l is
+// chunk P-1-l, M = Phi_{P-1-l}^T, v = w_{P-1-l}; the first link (the last chunk)
+// meets the zero state past the end of the signal, so it needs no matrix.
+//
+// Wave 0 chains; waves 1..3 load. Links come in batches of CB; batch j is loaded
+// by wave 1 + j % 3, which issues its global loads DEPTH = 3 batches before the
+// chain needs it and writes it to one of two LDS buffers in the iteration just
+// before. One barrier per batch keeps the four waves in step, so the chain never
+// waits for memory once the pipeline is full.
 constexpr int CB = 8;
 constexpr int SUMM = NP * NP + NP;
+// LDS layout of one link: 16 matrix rows padded to 17 doubles (row r at r*17,
+// so that 16 lanes reading one column, or one row, hit 16 distinct bank pairs),
+// then the 16-vector.
+constexpr int ROWP = NP + 1;
+constexpr int LINKP = NP * ROWP + NP;
+constexpr int CARRY_THREADS = 256;
+constexpr int DEPTH = 3;
 
 template <bool FWD>
-__global__ __launch_bounds__(64) void carry16(ChunkArgs<float> a) {
-  __shared__ double buf[2][CB][SUMM];
+__global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
+  __shared__ double buf[2][CB][LINKP];
   const int64_t b = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int64_t P = a.P;
   const int64_t links = P - 1;
+  const int64_t nbatch = (links + CB - 1) / CB;
   const double* summ = a.summ + b * links * SUMM;
   const double* wsum = FWD ? nullptr : a.wsum + b * links * NP;
   double* carry = a.carry + b * P * NP;
   constexpr int PER = CB * SUMM / 64;  // doubles per lane per batch
   double stage[PER];
-  auto fetch = [&](int64_t l0) {
+  auto fetch = [&](int64_t j) {
+    const int64_t l0 = j * CB;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int e = q * 64 + lane;
@@ -510,44 +531,49 @@ __global__ __launch_bounds__(64) void carry16(ChunkArgs<float> a) {
       stage[q] = v;
     }
   };
-  auto commit = [&](int bb) {
+  auto commit = [&](int64_t j) {
+    double(*dst)[LINKP] = buf[j & 1];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int e = q * 64 + lane;
-      buf[bb][e / SUMM][e % SUMM] = stage[q];
+      const int li = e / SUMM, off = e - li * SUMM;
+      dst[li][off < NP * NP ? off + off / NP : off + NP] = stage[q];
     }
   };
   double s = 0.0;
-  if (FWD && lane < NP && a.zi) s = (double)a.zi[b * NP + lane];
+  if (wave == 0 && FWD && lane < NP && a.zi) s = (double)a.zi[b * NP + lane];
   const int row = lane & 15;
-  if (links > 0) {
-    fetch(0);
-    commit(0);
-  }
-  for (int64_t l0 = 0, bb = 0; l0 < links; l0 += CB, bb ^= 1) {
-    if (l0 + CB < links) fetch(l0 + CB);
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this batch is in LDS
-    const int nb = (int)((links - l0) < CB ? (links - l0) : CB);
-    for (int li = 0; li < nb; ++li) {
-      const int64_t l = l0 + li;
-      if (lane < NP) carry[(FWD ? l : P - 1 - l) * NP + lane] = s;
-      double m[NP];
+  for (int64_t i = -DEPTH; i < nbatch; ++i) {
+    if (wave > 0) {
+      const int64_t jn = i + DEPTH;  // issue
+      if (jn < nbatch && 1 + jn % 3 == wave) fetch(jn);
+      const int64_t jc = i + 1;  // commit (loaded DEPTH - 1 iterations ago)
+      if (jc >= 0 && jc < nbatch && 1 + jc % 3 == wave) commit(jc);
+    } else if (i >= 0) {
+      const int64_t l0 = i * CB;
+      const int nb = (int)((links - l0) < CB ? (links - l0) : CB);
+      const double(*src)[LINKP] = buf[i & 1];
+      for (int li = 0; li < nb; ++li) {
+        const int64_t l = l0 + li;
+        if (lane < NP) carry[(FWD ? l : P - 1 - l) * NP + lane] = s;
+        double m[NP];
 #pragma unroll
-      for (int k = 0; k < NP; ++k) m[k] = FWD ? buf[bb][li][row * NP + k] : buf[bb][li][k * NP + row];
-      s = chain_link(s, m, buf[bb][li][NP * NP + row]);
+        for (int k = 0; k < NP; ++k) m[k] = FWD ? src[li][row * ROWP + k] : src[li][k * ROWP + row];
+        s = chain_link(s, m, src[li][NP * ROWP + row]);
+      }
     }
-    if (l0 + CB < links) commit(bb ^ 1);
+    __syncthreads();
   }
-  if (lane < NP) carry[(FWD ? P - 1 : 0) * NP + lane] = s;
+  if (wave == 0 && lane < NP) carry[(FWD ? P - 1 : 0) * NP + lane] = s;
 }
 
 }  // namespace c16
 
 int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st) {
   if (fwd)
-    hipLaunchKernelGGL(c16::carry16<true>, dim3((unsigned)a.B), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(c16::carry16<true>, dim3((unsigned)a.B), dim3(c16::CARRY_THREADS), 0, st, a);
   else
-    hipLaunchKernelGGL(c16::carry16<false>, dim3((unsigned)a.B), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(c16::carry16<false>, dim3((unsigned)a.B), dim3(c16::CARRY_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
