@@ -64,16 +64,16 @@ def test_config5_full_size_time_invariant():
     assert max(errs) <= 1e-5, errs
 
 
-@pytest.mark.parametrize("N,T", [(16, 5000), (4, 3000), (2, 70000)])
-def test_hip_graph_capture_replays(N, T):
+@pytest.mark.parametrize("N,T,B", [(16, 5000, 3), (16, 3000, 66), (4, 3000, 3), (2, 70000, 3)])
+def test_hip_graph_capture_replays(N, T, B):
     """The forward and the fused backward captured in one HIP graph: every replay
     (with eager work and synchronisation between replays) equals the eager
     result bit for bit."""
     rng = np.random.default_rng(N + T)
     from tests.gpu_util import stable_inputs
 
-    x, A, zi = stable_inputs(rng, 3, T, N, np.float32, kmax=0.8 if N == 16 else None, hop=240)
-    g = stable_inputs(rng, 3, T, 1, np.float32)[0]
+    x, A, zi = stable_inputs(rng, B, T, N, np.float32, kmax=0.8 if N == 16 else None, hop=240)
+    g = stable_inputs(rng, B, T, 1, np.float32)[0]
     X, AA, Z, G = dev(x), dev(A), dev(zi), dev(g)
 
     def step():

@@ -258,6 +258,23 @@ def test_backward_reuses_forward_summaries(T, chunk):
     assert all(torch.equal(a, b) for a, b in zip(again, first))
 
 
+def test_order16_large_batch_two_streams():
+    """Order 16, float32, B >= 64: the batch runs as two halves on the caller's
+    stream and a side stream; every output meets the FP64 oracle."""
+    rng = np.random.default_rng(66)
+    B, T = 67, 2500
+    x, A, zi = stable_inputs(rng, B, T, 16, np.float32, kmax=0.8, hop=240)
+    g = stable_inputs(rng, B, T, 1, np.float32)[0]
+    y, ws = ops().lpc_keep(dev(x), dev(A), dev(zi))
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    ref = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    assert rel(host(y), y64) <= 1e-5
+    for summary in (None, ws):
+        outs = ops().lpc_backward(dev(g), dev(A), dev(zi), y, True, True, True, 0, 0, summary)
+        for o, r in zip(outs, ref):
+            assert rel(host(o), r) <= 1e-5
+
+
 def test_autograd_order16_uses_kept_summaries():
     """LPC.apply on float32 order-16 inputs that require grad keeps the forward's
     workspace for the backward; the gradients match the FP64 oracle."""

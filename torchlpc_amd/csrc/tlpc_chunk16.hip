@@ -67,6 +67,15 @@ __device__ __forceinline__ bool yok(const float* zib, int u) {
   return u >= 0 || (zib != nullptr && -u - 1 < NP);
 }
 
+// Workgroup barrier that orders LDS only. __syncthreads() also fences global
+// memory, which waits for every load in flight (the prefetch rings); the waves
+// here exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int K>
 __device__ __forceinline__ void put(float4& o, int ql, float v) {
   // component K & 3 of lane K >> 2 of the quad
@@ -94,15 +103,15 @@ __device__ __forceinline__ void store16(float* vb, int tb, int T, int ql, float4
 
 // ---------------------------------------------------------------------------
 // Matrix rows. Lane kk of a 16-lane row reads A[t, kk+1] (ap = &A[t0, kk]);
-// cf is a 16-slot ring, loads run RD = 15 steps ahead (into the slot the
-// previous step freed).
-constexpr int RD = 15;
+// cf is a 16-slot ring: step J of a block consumes slot J and then reloads it
+// for step J of the next block, so nothing a block loads is used in the same
+// block (the order in which the compiler issues the loads does not matter).
 template <int J>
 __device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], const float* ap,
                                               int jb, int jlim) {
   if constexpr (J < NP) {
-    cf[(J + RD) & 15] = ap[min(jb + J + RD, jlim) * NP];
     r[J] = fwd_row_step<J>((double)cf[J], r);
+    cf[J] = ap[min(jb + J + NP, jlim) * NP];
     fwd_row_block<J + 1>(r, cf, ap, jb, jlim);
   }
 }
@@ -364,7 +373,7 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
     for (int i = 0; i < NP; ++i) r[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // y_{-1-kk} = 1
     float cf[NP];
 #pragma unroll
-    for (int j = 0; j < RD; ++j) cf[j] = opaque(ap[j * NP]);
+    for (int j = 0; j < NP; ++j) cf[j] = opaque(ap[min(j, jlim) * NP]);
     for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap, jb, jlim);
     double* out = a.summ + cid * (NP * NP + NP);
 #pragma unroll
@@ -562,7 +571,7 @@ __global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
         s = chain_link(s, m, src[li][NP * ROWP + row]);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (wave == 0 && lane < NP) carry[(FWD ? P - 1 : 0) * NP + lane] = s;
 }
