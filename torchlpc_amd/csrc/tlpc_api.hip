@@ -59,6 +59,23 @@ inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool single) 
   return (int)L;
 }
 
+// Time-invariant chunks: the summary cost no longer grows with the order (Phi is
+// built once per signal), so chunks are short enough for about 128K lanes in the
+// zero-state and rerun phases, at least 4*NP long, and at most 256 per signal
+// (the length of the carry chain). Summaries and carries are in the wide type,
+// so short float32 chunks keep their accuracy.
+inline int pick_chunk_ti(int64_t B, int64_t T, int np, int requested) {
+  const int blk = chunk_block(np);
+  int64_t L = requested > 0 ? requested : (B * T + 131071) / 131072;
+  if (requested <= 0) {
+    if (L < 4 * np) L = 4 * np;
+    if (L < (T + 255) / 256) L = (T + 255) / 256;
+  }
+  if (L < blk) L = blk;
+  if (L > (int64_t)1 << 24) L = (int64_t)1 << 24;
+  return (int)((L + blk - 1) / blk * blk);
+}
+
 // The per-signal fused kernel exists for the float32, order-16, time-varying
 // forward; it needs enough signals to occupy the chip (one workgroup each).
 inline bool signal_ok(int dtype, int64_t B, int n, int flags, bool backward, int neg) {
@@ -107,10 +124,12 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
     p.carry_off = 0;
     return p;
   }
-  p.L = pick_chunk(B, T, p.np, chunk_len, dtype == TLPC_F32 || dtype == TLPC_C64);
+  const bool ti = (flags & TLPC_TIME_INVARIANT) != 0;
+  p.L = ti ? pick_chunk_ti(B, T, p.np, chunk_len) : pick_chunk(B, T, p.np, chunk_len, dtype == TLPC_F32 || dtype == TLPC_C64);
   p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
   const size_t ws_es = wide_size(dtype);
-  const size_t summ = (size_t)B * (p.P > 1 ? p.P - 1 : 0) * (p.np * p.np + p.np) * ws_es;
+  // time-invariant summaries are the zero-state vectors only (Phi is per signal)
+  const size_t summ = (size_t)B * (p.P > 1 ? p.P - 1 : 0) * (ti ? p.np : p.np * p.np + p.np) * ws_es;
   const size_t carry = (size_t)B * p.P * p.np * ws_es;
   const size_t dapart =
       (backward && (flags & TLPC_TIME_INVARIANT) && (flags & TLPC_NEED_DA) && !(flags & TLPC_DA_FULL))

@@ -448,6 +448,71 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   }
 }
 
+// y_u of one signal for the adjoint's window: zi for u < 0 (k < n), zero past T.
+template <typename S>
+__device__ __forceinline__ S yval_at(const S* yb, const S* zib, int n, int64_t T, int64_t u) {
+  if (u >= 0) return (u < T) ? yb[u] : Num<S>::zero();
+  const int64_t k = -u - 1;
+  return (zib && k < n) ? zib[k] : Num<S>::zero();
+}
+
+// One lane's adjoint state and streams (see bwd_rerun).
+template <typename S, int NP>
+struct BwdLane {
+  using W = typename Wide<S>::type;
+  W ring[NP];  // pending sums, slot u mod NP for position u
+  S yr[NP];    // conj(y) window: slot u mod NP holds y_u for u in [t-NP, t-1] (data type, exact)
+  W dacc[NP];  // time-invariant da partials
+  const S* gb;
+  const S* Ab;
+  const S* yb;
+  const S* zib;
+  S* dxb;
+  S* dAb;
+  int64_t at, T;
+  bool need_da;
+};
+
+// Steps t = tb - JJ for JJ = 0..NP-1 (t mod NP = NP-1-JJ), unrolled by template
+// recursion so that every ring index is a compile-time constant (a runtime index
+// would put the float64 ring in scratch memory).
+template <typename S, int NP, int JJ>
+__device__ __forceinline__ void bwd_steps(const ChunkArgs<S>& a, BwdLane<S, NP>& st, int64_t tb) {
+  using W = typename Wide<S>::type;
+  if constexpr (JJ < NP) {
+    constexpr int j = NP - 1 - JJ;
+    const int64_t t = tb - JJ;
+    if (t < st.T) {
+      S row[NP];
+      load_row<S, NP>(row, a, st.Ab + t * st.at);
+      const W lam = convert<W>(st.gb[t]) + st.ring[j];
+      st.ring[j] = Num<W>::zero();
+#pragma unroll
+      for (int i = 1; i <= NP; ++i)
+        st.ring[(j - i) & (NP - 1)] =
+            Num<W>::fms(st.ring[(j - i) & (NP - 1)], Num<W>::conj(convert<W>(row[i - 1])), lam);
+      if (st.dxb) st.dxb[t] = convert<S>(lam);
+      if (st.need_da) {
+        if (st.dAb) {
+          S* drow = st.dAb + t * a.n;
+#pragma unroll
+          for (int i = 1; i <= NP; ++i)
+            if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
+        } else {
+#pragma unroll
+          for (int i = 1; i <= NP; ++i)
+            st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
+        }
+      }
+    }
+    if (st.need_da) {
+      // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
+      st.yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval_at(st.yb, st.zib, a.n, st.T, t - 1 - NP));
+    }
+    bwd_steps<S, NP, JJ + 1>(a, st, tb);
+  }
+}
+
 // Phase 3 (adjoint): one lane per chunk walks its samples backwards from the carry,
 // writes dx and dA (time-varying) or a per-chunk partial of da (time-invariant),
 // and chunk 0 writes dzi. Arithmetic in the wide type, as in the forward rerun.
@@ -460,78 +525,38 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   const Span sp = sig_span(a, b);
   const int64_t c = cid - sp.cbase;
   const int64_t t0 = c * a.L;
-  const int64_t T = sp.T;
-  W ring[NP];  // pending sums, slot u mod NP for position u
+  BwdLane<S, NP> st;
   const W* cr = a.carry + cid * NP;
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
-  const S* gb = a.x + sp.xo;  // upstream gradient
-  const S* Ab = a.A + sp.ao;
-  const S* yb = a.y + sp.xo;
-  const S* zib = a.zi ? a.zi + sp.zo : nullptr;
-  S* dxb = a.dx ? a.dx + sp.xo : nullptr;
-  S* dAb = a.dA ? a.dA + sp.dao : nullptr;
-  const bool need_da = a.dA != nullptr || a.dapart != nullptr;
-  // y ring: slot u mod NP holds y_u (zi for u < 0), for the window t-1 .. t-NP.
-  auto yval = [&](int64_t u) -> S {
-    if (u >= 0) return (u < T) ? yb[u] : Num<S>::zero();
-    const int64_t k = -u - 1;
-    return (zib && k < a.n) ? zib[k] : Num<S>::zero();
-  };
-  W yr[NP];
+  for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = cr[r];
+  st.gb = a.x + sp.xo;  // upstream gradient
+  st.Ab = a.A + sp.ao;
+  st.yb = a.y + sp.xo;
+  st.zib = a.zi ? a.zi + sp.zo : nullptr;
+  st.dxb = a.dx ? a.dx + sp.xo : nullptr;
+  st.dAb = a.dA ? a.dA + sp.dao : nullptr;
+  st.at = sp.at;
+  st.T = sp.T;
+  st.need_da = a.dA != nullptr || a.dapart != nullptr;
   const int64_t tstart = t0 + a.L - 1;  // first processed (possibly virtual) time
-  if (need_da) {
+  if (st.need_da) {
 #pragma unroll
     for (int i = 1; i <= NP; ++i)
-      yr[(NP - 1 - i + NP) & (NP - 1)] = convert<W>(Num<S>::conj(yval(tstart - i)));
+      st.yr[(NP - 1 - i + NP) & (NP - 1)] = Num<S>::conj(yval_at(st.yb, st.zib, a.n, st.T, tstart - i));
   }
-  W dacc[NP];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) dacc[i] = Num<W>::zero();
-
-  for (int jb = 0; jb < a.L; jb += NP) {
-#pragma unroll
-    for (int jj = 0; jj < NP; ++jj) {
-      const int j = NP - 1 - jj;  // t mod NP
-      const int64_t t = tstart - jb - jj;
-      if (t < T) {
-        S row[NP];
-        load_row<S, NP>(row, a, Ab + t * sp.at);
-        const W lam = convert<W>(gb[t]) + ring[j];
-        ring[j] = Num<W>::zero();
-#pragma unroll
-        for (int i = 1; i <= NP; ++i)
-          ring[(j - i) & (NP - 1)] =
-              Num<W>::fms(ring[(j - i) & (NP - 1)], Num<W>::conj(convert<W>(row[i - 1])), lam);
-        if (dxb) dxb[t] = convert<S>(lam);
-        if (need_da) {
-          if (dAb) {
-            S* drow = dAb + t * a.n;
-#pragma unroll
-            for (int i = 1; i <= NP; ++i)
-              if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * yr[(j - i) & (NP - 1)])));
-          } else {
-#pragma unroll
-            for (int i = 1; i <= NP; ++i) dacc[i - 1] = Num<W>::fms(dacc[i - 1], lam, yr[(j - i) & (NP - 1)]);
-          }
-        }
-      }
-      if (need_da) {
-        // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
-        yr[(j - 1) & (NP - 1)] = convert<W>(Num<S>::conj(yval(t - 1 - NP)));
-      }
-    }
-  }
+  for (int i = 0; i < NP; ++i) st.dacc[i] = Num<W>::zero();
+  for (int jb = 0; jb < a.L; jb += NP) bwd_steps<S, NP, 0>(a, st, tstart - jb);
   if (a.dapart) {
     W* dp = a.dapart + cid * NP;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) dp[i] = a.neg ? -dacc[i] : dacc[i];
+    for (int i = 0; i < NP; ++i) dp[i] = a.neg ? -st.dacc[i] : st.dacc[i];
   }
   if (c == 0 && a.dzi) {
     // pending sums for positions -1-r sit in slots NP-1-r
 #pragma unroll
     for (int r = 0; r < NP; ++r)
-      if (r < a.n) a.dzi[sp.zo + r] = convert<S>(ring[NP - 1 - r]);
+      if (r < a.n) a.dzi[sp.zo + r] = convert<S>(st.ring[NP - 1 - r]);
   }
 }
 
@@ -550,6 +575,134 @@ __global__ __launch_bounds__(256) void ti_da_reduce(ChunkArgs<S> a) {
 }
 
 // ----------------------------------------------------------------------------
+// Time-invariant coefficients (one row a[n] per signal, a_tstride == 0). Every
+// full chunk of a signal has the same transition matrix Phi = C^L (C the
+// companion matrix of a), so the summary phase only needs each chunk's
+// zero-state response (O(n) per sample, one lane per chunk), and the carry
+// kernel builds Phi once per signal from the n basis responses over L steps.
+// The adjoint's chunk map is Phi^H: the conjugated, transposed recurrence is the
+// adjoint of the forward one. Summaries hold only the NP-vector per chunk.
+
+template <typename S, int NP>
+__device__ __forceinline__ void load_coefs(typename Wide<S>::type (&co)[NP], const ChunkArgs<S>& a,
+                                           int64_t b, bool conj) {
+  using W = typename Wide<S>::type;
+  const S* arow = a.A + b * a.a_bstride;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const W v = i < a.n ? convert<W>(coef(a, arow[i])) : Num<W>::zero();
+    co[i] = conj ? Num<W>::conj(v) : v;
+  }
+}
+
+template <typename S, int NP, bool FWD>
+__global__ __launch_bounds__(256) void ti_zsum(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
+  const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t cps = a.P - 1;
+  if (item >= a.B * cps) return;
+  const int64_t b = item / cps;
+  const int64_t c = item - b * cps + (FWD ? 0 : 1);
+  const int64_t t0 = c * a.L;
+  W co[NP];
+  load_coefs<S, NP>(co, a, b, !FWD);
+  W ring[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) ring[r] = Num<W>::zero();
+  const S* inp = a.x + b * a.T;
+  if constexpr (FWD) {
+    // chunks 0..P-2 are whole: t < T throughout
+    for (int jb = 0; jb < a.L; jb += NP) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        W acc0 = convert<W>(inp[t0 + jb + j]), acc1 = Num<W>::zero();
+#pragma unroll
+        for (int i = 1; i <= NP; i += 2) {
+          acc0 = Num<W>::fms(acc0, co[i - 1], ring[(j - i) & (NP - 1)]);
+          if (i < NP) acc1 = Num<W>::fms(acc1, co[i], ring[(j - i - 1) & (NP - 1)]);
+        }
+        ring[j] = acc0 + acc1;
+      }
+    }
+  } else {
+    // descending; steps at or past T have g = 0 and leave the zero state as it is
+    for (int jb = 0; jb < a.L; jb += NP) {
+#pragma unroll
+      for (int jj = 0; jj < NP; ++jj) {
+        const int j = NP - 1 - jj;
+        const int64_t t = t0 + a.L - 1 - jb - jj;
+        const W lam = (t < a.T ? convert<W>(inp[t]) : Num<W>::zero()) + ring[j];
+        ring[j] = Num<W>::zero();
+#pragma unroll
+        for (int i = 1; i <= NP; ++i) ring[(j - i) & (NP - 1)] = Num<W>::fms(ring[(j - i) & (NP - 1)], co[i - 1], lam);
+      }
+    }
+  }
+  W* out = a.summ + item * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) out[r] = ring[NP - 1 - r];  // state at the chunk's far end
+}
+
+template <typename S, int NP, bool FWD>
+__global__ __launch_bounds__(64) void carry_ti(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
+  __shared__ W phi[NP][NP + 1];
+  __shared__ W sv[NP];
+  const int64_t b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t P = a.P;
+  if (lane < NP) {
+    // lane k: response to y_{-1-k} = 1 over one whole chunk (column k of Phi)
+    W co[NP];
+    load_coefs<S, NP>(co, a, b, false);
+    W ring[NP];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) ring[r] = (r == NP - 1 - lane) ? Num<W>::one() : Num<W>::zero();
+    for (int jb = 0; jb < a.L; jb += NP) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        W acc0 = Num<W>::zero(), acc1 = Num<W>::zero();
+#pragma unroll
+        for (int i = 1; i <= NP; i += 2) {
+          acc0 = Num<W>::fms(acc0, co[i - 1], ring[(j - i) & (NP - 1)]);
+          if (i < NP) acc1 = Num<W>::fms(acc1, co[i], ring[(j - i - 1) & (NP - 1)]);
+        }
+        ring[j] = acc0 + acc1;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NP; ++r) phi[r][lane] = ring[NP - 1 - r];
+  }
+  __syncthreads();
+  W s = Num<W>::zero();
+  if (FWD && lane < a.n && a.zi) s = convert<W>(a.zi[b * a.n + lane]);
+  W* carry = a.carry + b * P * NP;
+  const W* zs = a.summ + b * (P - 1) * NP;
+  for (int64_t step = 0; step < P; ++step) {
+    const int64_t c = FWD ? step : P - 1 - step;
+    if (lane < NP) {
+      carry[c * NP + lane] = s;
+      sv[lane] = s;
+    }
+    if (step == P - 1) break;
+    __syncthreads();
+    if (lane < NP) {
+      // forward chunk c is at slot c; adjoint chunk c (>= 1) at slot c-1
+      W acc0 = zs[(FWD ? c : c - 1) * NP + lane], acc1 = Num<W>::zero();
+#pragma unroll
+      for (int k = 0; k < NP; k += 2) {
+        const W m0 = FWD ? phi[lane][k] : Num<W>::conj(phi[k][lane]);
+        const W m1 = FWD ? phi[lane][k + 1 < NP ? k + 1 : k] : Num<W>::conj(phi[k + 1 < NP ? k + 1 : k][lane]);
+        acc0 = Num<W>::fma(acc0, m0, sv[k]);
+        if (k + 1 < NP) acc1 = Num<W>::fma(acc1, m1, sv[k + 1]);
+      }
+      s = acc0 + acc1;
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Host-side launch of the three phases.
 // Summary work items (see fill_items) and chunk count of a batch.
 template <typename S>
@@ -564,6 +717,14 @@ int64_t chunk_count(const ChunkArgs<S>& a) {
 template <typename S, int NP>
 int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
+  if (a.a_tstride == 0 && !a.rg) {
+    const int64_t nz = a.B * (a.P - 1);
+    if (nz > 0) hipLaunchKernelGGL((ti_zsum<S, NP, true>), dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((carry_ti<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
+    const int64_t nch = a.B * a.P;
+    hipLaunchKernelGGL((fwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+  }
   if (a.P > 1) {
     const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
     hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
@@ -585,6 +746,18 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
 template <typename S, int NP>
 int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
+  if (a.a_tstride == 0 && !a.rg) {
+    const int64_t nz = a.B * (a.P - 1);
+    if (nz > 0) hipLaunchKernelGGL((ti_zsum<S, NP, false>), dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((carry_ti<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
+    const int64_t nch = a.B * a.P;
+    hipLaunchKernelGGL((bwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+    if (a.dapart) {
+      const int64_t nd = a.B * a.n;
+      hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+  }
   if (a.P > 1) {
     const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
     hipLaunchKernelGGL((bwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
