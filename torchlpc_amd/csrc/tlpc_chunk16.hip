@@ -42,6 +42,16 @@ __device__ __forceinline__ float4 ld_row(const float* Ab, int t, int ql) {
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Keeps the memory instructions of a step where the source puts them: ALU work
+// may still move across, loads and stores may not. Without it the scheduler may
+// gather a block's ring reloads at the end of the loop body, right before the
+// next block consumes them, and the prefetch distance drops to nothing.
+__device__ __forceinline__ void pin_loads() {
+#ifndef TLPC_NO_PIN
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // The prologue fills of the prefetch rings pass through an empty asm statement.
 // A ring slot is a loop phi whose incoming values are both loads (the prologue's
 // and the previous iteration's); without the barrier InstCombine turns that phi
@@ -54,6 +64,16 @@ __device__ __forceinline__ float opaque(float v) {
 __device__ __forceinline__ float4 opaque(float4 v) {
   asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
   return v;
+}
+// A loop-carried state that comes from memory (the carries) passes through here
+// before the loop, which makes the kernel wait for it once there. Otherwise the
+// wait lands at its first use inside the loop, where the wait-count pass merges
+// it with the back edge and repeats it every block: a wait that also drains the
+// ring loads of the previous block.
+template <int K>
+__device__ __forceinline__ void settle(double (&v)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) asm volatile("" : "+v"(v[i]));
 }
 
 // y_u of one signal, with the initial conditions y_{-1-k} = zi[k] (k < 16) and
@@ -106,13 +126,16 @@ __device__ __forceinline__ void store16(float* vb, int tb, int T, int ql, float4
 // cf is a 16-slot ring: step J of a block consumes slot J and then reloads it
 // for step J of the next block, so nothing a block loads is used in the same
 // block (the order in which the compiler issues the loads does not matter).
+// The rows run whole chunks (t0 + L <= T), so the next block lies inside the
+// signal whenever it is used; its base np is clamped once per block to keep the
+// last prefetch in bounds, and the step loads are immediate offsets from it.
 template <int J>
-__device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], const float* ap,
-                                              int jb, int jlim) {
+__device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], const float* np) {
   if constexpr (J < NP) {
     r[J] = fwd_row_step<J>((double)cf[J], r);
-    cf[J] = ap[min(jb + J + NP, jlim) * NP];
-    fwd_row_block<J + 1>(r, cf, ap, jb, jlim);
+    cf[J] = np[J * NP];
+    pin_loads();
+    fwd_row_block<J + 1>(r, cf, np);
   }
 }
 
@@ -154,9 +177,24 @@ struct FwdIo {
   int T;
 };
 
-template <int JJ, bool STORE, bool VEC, typename V>
+// Whole-block ring reloads go out RG rows at a time (RG * 64 contiguous bytes per
+// quad): a stream's consecutive cache lines then reach the memory controller
+// together instead of a step apart.
+#ifndef TLPC_RG
+#define TLPC_RG 4
+#endif
+constexpr int RG = TLPC_RG;
+static_assert(PF % RG == 0, "reload groups tile the ring");
+
+// WHOLE: every 16-step block that the run uses lies wholly inside the signal
+// (the run covers whole blocks: a full chunk, or T a multiple of 16). The ring
+// reloads are then immediate offsets from one base per block (nb: lane ql's
+// float4 of the next block's first row), clamped once per block so that the
+// prefetch past the run stays in the signal; otherwise every step clamps its row.
+template <int JJ, bool STORE, bool VEC, bool WHOLE, typename V>
 __device__ __forceinline__ void fwd_q16(V (&R)[4], float4 (&cq)[PF], float (&xq)[PF], float4& out,
-                                        const float4& xc, const FwdIo& io, int tb, int ql) {
+                                        const float4& xc, const FwdIo& io, int tb, int ql,
+                                        const float4* nb) {
   if constexpr (JJ < 16) {
     // the step consumes its ring slot before the slot is reloaded (for step
     // JJ + PF), so the old and new values never overlap and the slot stays in
@@ -166,14 +204,23 @@ __device__ __forceinline__ void fwd_q16(V (&R)[4], float4 (&cq)[PF], float (&xq)
     else xv = xq[JJ % PF];
     const V y = fwd_quad_step<JJ & 3>(R, cq[JJ % PF], xv, ql);
     const int tn = min(tb + JJ + PF, io.T - 1);
-    cq[JJ % PF] = ld_row(io.Ab, tn, ql);
+    if constexpr (WHOLE) {
+      // RG consumed slots at a time: RG consecutive rows, issued together
+      if constexpr (JJ % RG == RG - 1) {
+#pragma unroll
+        for (int k = JJ - RG + 1; k <= JJ; ++k) cq[k] = nb[k * (NP / 4)];
+      }
+    } else {
+      cq[JJ % PF] = ld_row(io.Ab, tn, ql);
+    }
     if constexpr (!VEC) xq[JJ % PF] = io.xb[tn];
+    pin_loads();
     if constexpr (STORE) put<JJ>(out, ql, (float)y);
-    fwd_q16<JJ + 1, STORE, VEC>(R, cq, xq, out, xc, io, tb, ql);
+    fwd_q16<JJ + 1, STORE, VEC, WHOLE>(R, cq, xq, out, xc, io, tb, ql, nb);
   }
 }
 
-template <bool STORE, bool VEC, typename V>
+template <bool STORE, bool VEC, bool WHOLE, typename V>
 __device__ __forceinline__ void fwd_quad_run(V (&R)[4], const FwdIo& io, int t0, int len, int ql) {
   float4 cq[PF];
   float xq[PF];
@@ -185,12 +232,15 @@ __device__ __forceinline__ void fwd_quad_run(V (&R)[4], const FwdIo& io, int t0,
   }
   float4 xc = make_float4(0.f, 0.f, 0.f, 0.f);
   if constexpr (VEC) xc = opaque(ld4(io.xb, t0, ql, io.T));
+  settle(R);
   for (int jb = 0; jb < len; jb += 16) {
     const int tb = t0 + jb;
     float4 xn = xc;
     if constexpr (VEC) xn = ld4(io.xb, tb + 16, ql, io.T);  // used by the next block
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-    fwd_q16<0, STORE, VEC>(R, cq, xq, out, xc, io, tb, ql);
+    const float4* nb = nullptr;
+    if constexpr (WHOLE) nb = reinterpret_cast<const float4*>(io.Ab + min(tb + 16, io.T - 16) * NP) + ql;
+    fwd_q16<0, STORE, VEC, WHOLE>(R, cq, xq, out, xc, io, tb, ql, nb);
     if constexpr (STORE) store16(io.yb, tb, io.T, ql, out);
     xc = xn;
   }
@@ -218,11 +268,14 @@ struct AdjVec {
   float4 g, ycur, ynext;
 };
 
-template <int JJ, bool OUT, bool DA, bool VEC>
+// WHOLE as for the forward run, descending: pb is lane ql's float4 of the first
+// row of block tb - 16 (clamped into the signal), and with T a multiple of 16 a
+// block is either inside the signal or wholly past its end (no live step).
+template <int JJ, bool OUT, bool DA, bool VEC, bool WHOLE>
 __device__ __forceinline__ void adj_q16(double (&P)[4], double& now, float (&Y)[4],
                                         float4 (&cq)[PF], float (&gq)[PF], float (&yq)[PF],
                                         float4& dxo, const AdjVec& vb, const AdjIo& io, int tb,
-                                        int ql) {
+                                        int ql, const float4* pb) {
   if constexpr (JJ < 16) {
     constexpr int J = (3 - JJ) & 3;  // slot of t = tb + 15 - JJ (tb is a multiple of 16)
     const int t = tb + 15 - JJ;
@@ -239,12 +292,20 @@ __device__ __forceinline__ void adj_q16(double (&P)[4], double& now, float (&Y)[
     const double lam = adj_quad_step<J>(P, now, cq[JJ % PF], gv, ql, live);
     {
       const int tc = clampi(t - PF, 0, io.T - 1);
-      cq[JJ % PF] = ld_row(io.Ab, tc, ql);
+      if constexpr (WHOLE) {
+        if constexpr (JJ % RG == RG - 1) {
+#pragma unroll
+          for (int k = JJ - RG + 1; k <= JJ; ++k) cq[k] = pb[(15 - k) * (NP / 4)];
+        }
+      } else {
+        cq[JJ % PF] = ld_row(io.Ab, tc, ql);
+      }
       if constexpr (!VEC) {
         gq[JJ % PF] = io.gb[tc];
         if constexpr (DA) yq[JJ % PF] = *yaddr(io.yb, io.zib, t - PF - 17, io.T);
       }
     }
+    pin_loads();
     if constexpr (OUT) {
       const float lf = (float)lam;
       put<15 - JJ>(dxo, ql, lf);
@@ -264,7 +325,7 @@ __device__ __forceinline__ void adj_q16(double (&P)[4], double& now, float (&Y)[
         Y[S] = (ql == 3) ? ynew : down;  // lane 3 takes y_{t-17}
       }
     }
-    adj_q16<JJ + 1, OUT, DA, VEC>(P, now, Y, cq, gq, yq, dxo, vb, io, tb, ql);
+    adj_q16<JJ + 1, OUT, DA, VEC, WHOLE>(P, now, Y, cq, gq, yq, dxo, vb, io, tb, ql, pb);
   }
 }
 
@@ -282,7 +343,42 @@ __device__ __forceinline__ float4 yblock_fix(const float4& raw, const float4& zr
   return r;
 }
 
-template <bool OUT, bool DA, bool VEC>
+// One 16-step block of the adjoint run. The float4 blocks of g and y live in
+// two-slot rings indexed by the block's parity PAR (block k of the run, k = 0 at
+// the top): gs[PAR] holds g of this block, ys[PAR ^ 1] the y block below it and
+// ys[PAR] the one below that (read by the last step only). Each slot is reloaded
+// in place as soon as its last reader is done, for the block two (g) or three (y)
+// further down; with the parity a template argument no value moves between
+// registers at the loop latch, and the latch needs no wait for loads in flight.
+template <int PAR, bool OUT, bool DA, bool VEC, bool WHOLE>
+__device__ __forceinline__ void adj_block(double (&P)[4], double& now, float (&Y)[4],
+                                          float4 (&cq)[PF], float (&gq)[PF], float (&yq)[PF],
+                                          float4 (&gs)[2], float4 (&ys)[2], const float4& zrev,
+                                          const AdjIo& io, int tb, int ql) {
+  AdjVec vb;
+  if constexpr (VEC) {
+    vb.g = gs[PAR];
+    if constexpr (DA) {
+      vb.ycur = yblock_fix(ys[PAR ^ 1], zrev, tb - 16);
+      vb.ynext = yblock_fix(ys[PAR], zrev, tb - 32);
+    }
+  }
+  float4 dxo = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4* pb = nullptr;
+  if constexpr (WHOLE) pb = reinterpret_cast<const float4*>(io.Ab + clampi(tb - 16, 0, io.T - 16) * NP) + ql;
+  adj_q16<0, OUT, DA, VEC, WHOLE>(P, now, Y, cq, gq, yq, dxo, vb, io, tb, ql, pb);
+  if constexpr (OUT) {
+    if (io.dxb) store16(io.dxb, tb, io.T, ql, dxo);
+  }
+  if constexpr (VEC) {
+    gs[PAR] = ld4(io.gb, tb - 32, ql, io.T);
+    if constexpr (DA) ys[PAR ^ 1] = ld4(io.yb, tb - 48, ql, io.T);
+  }
+}
+
+// The run takes its blocks in pairs (parities 0, 1); an odd block count ends
+// with one more parity-0 block.
+template <bool OUT, bool DA, bool VEC, bool WHOLE>
 __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const AdjIo& io, int t0,
                                              int L, int ql) {
   const int tstart = t0 + L - 1;
@@ -309,12 +405,14 @@ __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const 
     }
   }
   const int tb0 = t0 + L - 16;
-  float4 gc = make_float4(0.f, 0.f, 0.f, 0.f), yc = gc, yn = gc, zrev = gc;
+  float4 gs[2], ys[2], zrev = make_float4(0.f, 0.f, 0.f, 0.f);
+  gs[0] = gs[1] = ys[0] = ys[1] = zrev;
   if constexpr (VEC) {
-    gc = opaque(ld4(io.gb, tb0, ql, io.T));
+    gs[0] = opaque(ld4(io.gb, tb0, ql, io.T));
+    gs[1] = opaque(ld4(io.gb, tb0 - 16, ql, io.T));
     if constexpr (DA) {
-      yc = opaque(ld4(io.yb, tb0 - 16, ql, io.T));
-      yn = opaque(ld4(io.yb, tb0 - 32, ql, io.T));
+      ys[1] = opaque(ld4(io.yb, tb0 - 16, ql, io.T));
+      ys[0] = opaque(ld4(io.yb, tb0 - 32, ql, io.T));
       // zi as y_{-16..-1}: lane ql, component m holds y_{-16+4ql+m} = zi[15-4ql-m]
       if (io.zib) {
         const float4 zv = *reinterpret_cast<const float4*>(io.zib + 4 * ql);
@@ -326,33 +424,18 @@ __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const 
       }
     }
   }
-  for (int tb = tb0; tb >= t0; tb -= 16) {
-    AdjVec vb;
-    float4 gn = gc, yn2 = yn;
-    if constexpr (VEC) {
-      gn = ld4(io.gb, tb - 16, ql, io.T);  // used by the next block
-      vb.g = gc;
-      if constexpr (DA) {
-        yn2 = ld4(io.yb, tb - 48, ql, io.T);
-        vb.ycur = yblock_fix(yc, zrev, tb - 16);
-        vb.ynext = yblock_fix(yn, zrev, tb - 32);
-      }
-    }
-    float4 dxo = make_float4(0.f, 0.f, 0.f, 0.f);
-    adj_q16<0, OUT, DA, VEC>(P, now, Y, cq, gq, yq, dxo, vb, io, tb, ql);
-    if constexpr (OUT) {
-      if (io.dxb) store16(io.dxb, tb, io.T, ql, dxo);
-    }
-    gc = gn;
-    yc = yn;
-    yn = yn2;
+  int tb = tb0;
+  for (; tb - 16 >= t0; tb -= 32) {
+    adj_block<0, OUT, DA, VEC, WHOLE>(P, now, Y, cq, gq, yq, gs, ys, zrev, io, tb, ql);
+    adj_block<1, OUT, DA, VEC, WHOLE>(P, now, Y, cq, gq, yq, gs, ys, zrev, io, tb - 16, ql);
   }
+  if (tb >= t0) adj_block<0, OUT, DA, VEC, WHOLE>(P, now, Y, cq, gq, yq, gs, ys, zrev, io, tb, ql);
 }
 
 // ---------------------------------------------------------------------------
 // Forward summaries: chunks c = 0..P-2 (full length L).
 template <bool WITH_Z, bool VEC>
-__global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
+__global__ __launch_bounds__(SUM_THREADS, WITH_Z ? (VEC ? 5 : 4) : 5) void sum_fwd(ChunkArgs<float> a) {
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t cps = a.P - 1;
@@ -374,7 +457,8 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
     float cf[NP];
 #pragma unroll
     for (int j = 0; j < NP; ++j) cf[j] = opaque(ap[min(j, jlim) * NP]);
-    for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap, jb, jlim);
+    // jlim >= L - 1 >= NP - 1: the clamped base keeps all NP rows in the signal
+    for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap + min(jb + NP, jlim - (NP - 1)) * NP);
     double* out = a.summ + cid * (NP * NP + NP);
 #pragma unroll
     for (int rr = 0; rr < NP; ++rr) out[rr * NP + kk] = r[NP - 1 - rr];
@@ -386,7 +470,7 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
     const int c = (int)(cid - b * cps);
     double R[4] = {0.0, 0.0, 0.0, 0.0};
     const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, T};
-    fwd_quad_run<false, VEC>(R, io, c * L, L, ql);
+    fwd_quad_run<false, VEC, true>(R, io, c * L, L, ql);  // full chunks: whole blocks
     double* out = a.summ + cid * (NP * NP + NP) + NP * NP;
 #pragma unroll
     for (int m = 0; m < 4; ++m) out[4 * ql + m] = R[3 - m];  // z[4ql+m] = y_{L-1-4ql-m}
@@ -396,7 +480,9 @@ __global__ __launch_bounds__(SUM_THREADS) void sum_fwd(ChunkArgs<float> a) {
 // Adjoint zero-state responses: chunks c = 1..P-1, one quad each (the last may
 // be partial; its steps past the end are no-ops of the zero state).
 constexpr int WSUM_THREADS = 256;
-template <bool VEC>
+// Stream modes of the kernels below: 0 per-step loads, 1 float4 blocks of the
+// per-sample streams (VEC), 2 VEC plus whole-block A rows (T a multiple of 16).
+template <int MODE>
 __global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
   const int64_t cps = a.P - 1;
   const int64_t cid = ((int64_t)blockIdx.x * WSUM_THREADS + threadIdx.x) >> 2;
@@ -407,7 +493,7 @@ __global__ __launch_bounds__(WSUM_THREADS) void sum_adj(ChunkArgs<float> a) {
   double P[4] = {0.0, 0.0, 0.0, 0.0};
   double now = 0.0;
   const AdjIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, nullptr, nullptr, nullptr, (int)a.T};
-  adj_quad_run<false, false, VEC>(P, now, io, c * a.L, a.L, ql);
+  adj_quad_run<false, false, (MODE >= 1), (MODE == 2)>(P, now, io, c * a.L, a.L, ql);
   // w[0] = now (lane 0); w[4ql+1+m] = P[(2-m) & 3]
   double* out = a.wsum + cid * NP;
   if (ql == 0) out[0] = now;
@@ -427,7 +513,7 @@ __device__ __forceinline__ int64_t rerun_chunk(const ChunkArgs<float>& a) {
   return a.B * a.P - 1 - (((int64_t)blockIdx.x * RERUN_THREADS + threadIdx.x) >> 2);
 }
 
-template <bool VEC>
+template <int MODE>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
   const int64_t cid = rerun_chunk(a);
   if (cid < 0) return;
@@ -442,12 +528,12 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_fwd(ChunkArgs<float> a) {
 #pragma unroll
   for (int m = 0; m < 4; ++m) R[3 - m] = cr[4 * ql + m];  // y_{t0-1-4ql-m}, slot 3-m
   const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, a.y + b * a.T, T};
-  fwd_quad_run<true, VEC>(R, io, t0, len, ql);
+  fwd_quad_run<true, (MODE >= 1), (MODE == 2)>(R, io, t0, len, ql);
 }
 
 // Adjoint rerun: one quad per chunk, descending from the chunk's last step (steps
 // past the end of the last chunk are no-ops of its zero carry).
-template <bool DA, bool VEC>
+template <bool DA, int MODE>
 __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
   const int64_t cid = rerun_chunk(a);
   if (cid < 0) return;
@@ -472,7 +558,7 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
                  a.dx ? a.dx + b * a.T : nullptr,
                  a.dA ? a.dA + b * a.T * NP : nullptr,
                  T};
-  adj_quad_run<true, DA, VEC>(P, now, io, c * L, L, ql);
+  adj_quad_run<true, DA, (MODE >= 1), (MODE == 2)>(P, now, io, c * L, L, ql);
   if (c == 0 && a.dzi) {
     // pending of positions -1-r: r = 0 in `now` (lane 0), r = 4ql+1+m in slot (2-m)&3
     float* dz = a.dzi + b * NP;
@@ -592,6 +678,12 @@ static bool vec_ok(const ChunkArgs<float>& a) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   return a.T % 4 == 0 && al(a.x) && (!a.y || al(a.y)) && (!a.zi || al(a.zi));
 }
+// Stream mode of the quad kernels: whole 16-step blocks need T % 16 == 0 (chunk
+// starts are multiples of 16), so every block is inside the signal or past it.
+static int stream_mode(const ChunkArgs<float>& a) {
+  if (!vec_ok(a)) return 0;
+  return a.T % 16 == 0 ? 2 : 1;
+}
 
 int launch_chunk16_fwd_summary(ChunkArgs<float> a, bool with_z, hipStream_t st) {
   const int64_t n = a.B * (a.P - 1);
@@ -608,33 +700,40 @@ int launch_chunk16_fwd_summary(ChunkArgs<float> a, bool with_z, hipStream_t st) 
 int launch_chunk16_adj_summary(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * (a.P - 1) * 4;
   const dim3 grid((unsigned)((lanes + c16::WSUM_THREADS - 1) / c16::WSUM_THREADS));
-  if (lanes > 0 && vec_ok(a))
-    hipLaunchKernelGGL(c16::sum_adj<true>, grid, dim3(c16::WSUM_THREADS), 0, st, a);
-  else if (lanes > 0)
-    hipLaunchKernelGGL(c16::sum_adj<false>, grid, dim3(c16::WSUM_THREADS), 0, st, a);
+  if (lanes <= 0) return 0;
+  switch (stream_mode(a)) {
+    case 2: hipLaunchKernelGGL(c16::sum_adj<2>, grid, dim3(c16::WSUM_THREADS), 0, st, a); break;
+    case 1: hipLaunchKernelGGL(c16::sum_adj<1>, grid, dim3(c16::WSUM_THREADS), 0, st, a); break;
+    default: hipLaunchKernelGGL(c16::sum_adj<0>, grid, dim3(c16::WSUM_THREADS), 0, st, a); break;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * a.P * 4;
   const dim3 grid((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS));
-  if (vec_ok(a))
-    hipLaunchKernelGGL(c16::rerun_fwd<true>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
-  else
-    hipLaunchKernelGGL(c16::rerun_fwd<false>, grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  switch (stream_mode(a)) {
+    case 2: hipLaunchKernelGGL(c16::rerun_fwd<2>, grid, dim3(c16::RERUN_THREADS), 0, st, a); break;
+    case 1: hipLaunchKernelGGL(c16::rerun_fwd<1>, grid, dim3(c16::RERUN_THREADS), 0, st, a); break;
+    default: hipLaunchKernelGGL(c16::rerun_fwd<0>, grid, dim3(c16::RERUN_THREADS), 0, st, a); break;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st) {
   const int64_t lanes = a.B * a.P * 4;
   const dim3 grid((unsigned)((lanes + c16::RERUN_THREADS - 1) / c16::RERUN_THREADS));
-  const bool v = vec_ok(a);
-  if (a.dA && v)
-    hipLaunchKernelGGL((c16::rerun_adj<true, true>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  const int m = stream_mode(a);
+  if (a.dA && m == 2)
+    hipLaunchKernelGGL((c16::rerun_adj<true, 2>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else if (a.dA && m == 1)
+    hipLaunchKernelGGL((c16::rerun_adj<true, 1>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
   else if (a.dA)
-    hipLaunchKernelGGL((c16::rerun_adj<true, false>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
-  else if (v)
-    hipLaunchKernelGGL((c16::rerun_adj<false, true>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+    hipLaunchKernelGGL((c16::rerun_adj<true, 0>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else if (m == 2)
+    hipLaunchKernelGGL((c16::rerun_adj<false, 2>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+  else if (m == 1)
+    hipLaunchKernelGGL((c16::rerun_adj<false, 1>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
   else
-    hipLaunchKernelGGL((c16::rerun_adj<false, false>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
+    hipLaunchKernelGGL((c16::rerun_adj<false, 0>), grid, dim3(c16::RERUN_THREADS), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

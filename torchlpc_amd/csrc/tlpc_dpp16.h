@@ -22,36 +22,38 @@ constexpr int PF = 16;  // rows in flight per quad lane (one 16-step block)
 // ---------------------------------------------------------------------------
 // Forward matrix step (slot J = t mod 16 of the output ring r, r[u mod 16] = y_u):
 //   returns -sum_{i=1..16} A[t,i] * r[(J - i) & 15]
-// Two accumulators interleave even and odd lags. One asm statement, opened by
-// s_nop 1: the two wait states a DPP source needs after the VALU conversion
-// that produced it.
+// One accumulator chain, oldest lag first: the newest value r[J-1], which the
+// previous step produced, is read last, and no closing add is needed (two chains
+// would cost a second zeroed accumulator and an add per step; the waves sharing a
+// SIMD hide the longer chain). One asm statement, opened by s_nop 1: the two wait
+// states a DPP source needs after the VALU conversion that produced it.
 template <int J>
 __device__ __forceinline__ double fwd_row_step(double a, const double (&r)[NP]) {
-  double acc0 = 0.0, acc1 = 0.0;
+  double acc = 0.0;
   asm("s_nop 1\n\t"
-      "v_fmac_f64_dpp %0, -%2, %3 row_newbcast:15 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %4 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %5 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %6 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %7 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %8 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %9 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %14 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %15 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %16 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%2, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, -%2, %18 row_newbcast:0 row_mask:0xf bank_mask:0xf"
-      : "+v"(acc0), "+v"(acc1)
+      "v_fmac_f64_dpp %0, -%1, %2 row_newbcast:15 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %3 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %4 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %5 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %7 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %8 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %9 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %10 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %11 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %12 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %13 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %14 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %15 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %16 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%1, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc)
       : "v"(a), "v"(r[(J - 16) & 15]), "v"(r[(J - 15) & 15]), "v"(r[(J - 14) & 15]),
         "v"(r[(J - 13) & 15]), "v"(r[(J - 12) & 15]), "v"(r[(J - 11) & 15]),
         "v"(r[(J - 10) & 15]), "v"(r[(J - 9) & 15]), "v"(r[(J - 8) & 15]), "v"(r[(J - 7) & 15]),
         "v"(r[(J - 6) & 15]), "v"(r[(J - 5) & 15]), "v"(r[(J - 4) & 15]), "v"(r[(J - 3) & 15]),
         "v"(r[(J - 2) & 15]), "v"(r[(J - 1) & 15]));
-  return acc0 + acc1;
+  return acc;
 }
 
 // Carry link on lanes 0-15 (lane r owns s_r): returns z_r + sum_k M[r][k] s_k in
