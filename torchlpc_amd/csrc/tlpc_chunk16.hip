@@ -648,13 +648,24 @@ __global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
       const int64_t l0 = i * CB;
       const int nb = (int)((links - l0) < CB ? (links - l0) : CB);
       const double(*src)[LINKP] = buf[i & 1];
-      for (int li = 0; li < nb; ++li) {
-        const int64_t l = l0 + li;
-        if (lane < NP) carry[(FWD ? l : P - 1 - l) * NP + lane] = s;
-        double m[NP];
+      // the LDS reads of link li + 1 go out before the chain step of link li, so
+      // the chain waits for LDS only once per batch (links past nb read slots of
+      // the batch buffer that nothing uses)
+      double m[2][NP], zv[2];
+      auto ldm = [&](int li, double(&mm)[NP], double& z) {
 #pragma unroll
-        for (int k = 0; k < NP; ++k) m[k] = FWD ? src[li][row * ROWP + k] : src[li][k * ROWP + row];
-        s = chain_link(s, m, src[li][NP * ROWP + row]);
+        for (int k = 0; k < NP; ++k) mm[k] = FWD ? src[li][row * ROWP + k] : src[li][k * ROWP + row];
+        z = src[li][NP * ROWP + row];
+      };
+      ldm(0, m[0], zv[0]);
+#pragma unroll
+      for (int li = 0; li < CB; ++li) {
+        if (li + 1 < CB) ldm(li + 1, m[(li + 1) & 1], zv[(li + 1) & 1]);
+        if (li < nb) {
+          const int64_t l = l0 + li;
+          if (lane < NP) carry[(FWD ? l : P - 1 - l) * NP + lane] = s;
+          s = chain_link(s, m[li & 1], zv[li & 1]);
+        }
       }
     }
     lds_barrier();
