@@ -59,6 +59,37 @@ inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool single) 
   return (int)L;
 }
 
+// Compute units of the current device (cached per device).
+inline int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
+// Order-16 float32 chunks (the quad kernels): at most 64 chunks per compute
+// unit, i.e. one 256-thread rerun workgroup per CU and one round of summary
+// workgroups. These are fewer, longer chunks than the generic rule gives: the
+// reruns stream A and dA for every chunk at once and fewer streams use the memory
+// better, and the carry chains are shorter. The rerun time follows the busiest
+// CU (a CU given two workgroups takes twice as long), so the chunks per signal
+// are chosen first, P = 64 * CUs / B, and the length rounded up to 16 samples
+// from there (which can only lower P). At least 16 * 16 samples.
+inline int pick_chunk16(int64_t B, int64_t T, int requested) {
+  if (requested > 0) return pick_chunk(B, T, 16, requested, true);
+  int64_t P = 64 * (int64_t)device_cus() / (B > 0 ? B : 1);
+  if (P < 1) P = 1;
+  int64_t L = (T + P - 1) / P;
+  if (L < 256) L = 256;
+  if (L > 4096) L = 4096;
+  return (int)((L + 15) / 16 * 16);
+}
+
 // Time-invariant chunks: the summary cost no longer grows with the order (Phi is
 // built once per signal), so chunks are short enough for about 128K lanes in the
 // zero-state and rerun phases, at least 4*NP long, and at most 256 per signal
@@ -125,7 +156,12 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
     return p;
   }
   const bool ti = (flags & TLPC_TIME_INVARIANT) != 0;
-  p.L = ti ? pick_chunk_ti(B, T, p.np, chunk_len) : pick_chunk(B, T, p.np, chunk_len, dtype == TLPC_F32 || dtype == TLPC_C64);
+  if (ti)
+    p.L = pick_chunk_ti(B, T, p.np, chunk_len);
+  else if (chunk16_ok(dtype, n, flags, neg, 256, T))
+    p.L = pick_chunk16(B, T, chunk_len);
+  else
+    p.L = pick_chunk(B, T, p.np, chunk_len, dtype == TLPC_F32 || dtype == TLPC_C64);
   p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
   const size_t ws_es = wide_size(dtype);
   // time-invariant summaries are the zero-state vectors only (Phi is per signal)
