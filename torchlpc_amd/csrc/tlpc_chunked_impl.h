@@ -410,25 +410,21 @@ __device__ __forceinline__ void load_row(S (&row)[NP], const ChunkArgs<S>& a, co
 }
 
 // ----------------------------------------------------------------------------
-// Phase 3 (reruns): one lane per chunk re-filters its samples from the carry.
-// The recurrence runs in the wide type (float64 for float32 data), so the only
-// single-precision rounding of an output is its final store.
+// Phase 3 (reruns), time-invariant filters: one lane per chunk re-filters its
+// samples from the carry, in the wide type (float64 for float32 data), so the
+// only single-precision rounding of an output is its final store.
 //
-// The lane's inputs are prefetched: the per-sample streams (x or g, and y for
-// dA) DS steps ahead and the A rows DR steps ahead (as many rows as fit about 64
-// registers), in rings whose slots are consumed before they are reloaded. Every
-// load is unconditional, at an index clamped into the signal, and values that
-// must count as zero are selected after the load; a load under a branch would
-// leave the compiler a phi that it waits for at once. Steps past the end of a
-// signal compute on clamped data and store nothing (forward) or are no-ops of a
-// zero lambda (adjoint). Time-invariant coefficients are loaded once.
+// The coefficients are loaded once. The per-sample streams (x or g, and y for
+// da) are prefetched DS steps ahead in rings whose slots are consumed before
+// they are reloaded. Every load is unconditional, at an index clamped into the
+// signal, and values that must count as zero are selected after the load; a
+// load under a branch would leave the compiler a phi that it waits for at once.
+// Steps past the end of a signal compute on clamped data and store nothing
+// (forward) or are no-ops of a zero lambda (adjoint).
 template <typename S, int NP>
 struct RerunCfg {
   static constexpr int K = NP < 16 ? 16 : NP;  // steps per unrolled block: a multiple of NP and DS
   static constexpr int DS = 16;
-  static constexpr int WORDS = (int)(sizeof(S) / 4);
-  static constexpr int DR0 = 32 / (NP * WORDS);
-  static constexpr int DR = DR0 >= 16 ? 16 : (DR0 >= 8 ? 8 : (DR0 >= 4 ? 4 : (DR0 >= 2 ? 2 : 1)));
 };
 
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
@@ -438,20 +434,17 @@ __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
 // Keeps a step's loads where the source puts them (see tlpc_chunk16.hip).
 __device__ __forceinline__ void pin_step() { __builtin_amdgcn_sched_barrier(0); }
 
-// The NP coefficient slots of row t; slots at or past the order read a valid
-// element that coef_at() turns into zero.
+// The NP coefficients as the recurrence uses them: -A for the first-order scan,
+// zero at and past the order (those slots read a valid element first).
 template <typename S, int NP>
-__device__ __forceinline__ void ld_row_any(S (&row)[NP], const S* Ab, int64_t at, int64_t t, int n) {
-  const S* p = Ab + t * at;
-#pragma unroll
-  for (int i = 0; i < NP; ++i) row[i] = p[i < n ? i : n - 1];
-}
-template <typename S, int NP>
-__device__ __forceinline__ typename Wide<S>::type coef_at(const ChunkArgs<S>& a, const S (&row)[NP], int i) {
+__device__ __forceinline__ void load_coefs_w(typename Wide<S>::type (&cw)[NP], const ChunkArgs<S>& a, const S* p) {
   using W = typename Wide<S>::type;
-  const W v = convert<W>(row[i]);
-  const W c = a.neg ? -v : v;
-  return i < a.n ? c : Num<W>::zero();
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const W v = convert<W>(p[i < a.n ? i : a.n - 1]);
+    const W c = a.neg ? -v : v;
+    cw[i] = i < a.n ? c : Num<W>::zero();
+  }
 }
 
 template <typename T>
@@ -466,87 +459,66 @@ __device__ __forceinline__ T opaque_v(T v) {
 }
 
 template <typename S, int NP>
-struct FwdLane {
+struct FwdLaneTI {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
-  W ring[NP];            // y_u in slot u mod NP
-  S xs[RC::DS];          // x_t in slot t mod DS
-  S rows[RC::DR][NP];    // row t in slot t mod DR (time-varying)
-  S row0[NP];            // the row (time-invariant)
+  W ring[NP];    // y_u in slot u mod NP
+  W cw[NP];      // coefficients
+  S xs[RC::DS];  // x_t in slot t mod DS
   const S* xb;
-  const S* Ab;
   S* yb;
-  int64_t at, T, tend;
+  int64_t T, tend;
 };
 
-template <typename S, int NP, bool TI, int JJ>
-__device__ __forceinline__ void fwd_steps(const ChunkArgs<S>& a, FwdLane<S, NP>& st, int64_t tb) {
+template <typename S, int NP, int JJ>
+__device__ __forceinline__ void fwd_steps_ti(FwdLaneTI<S, NP>& st, int64_t tb) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
   if constexpr (JJ < RC::K) {
     constexpr int j = JJ % NP;  // t mod NP: blocks start at multiples of NP
     const int64_t t = tb + JJ;
     W acc = convert<W>(st.xs[JJ % RC::DS]);
-    if constexpr (TI) {
 #pragma unroll
-      for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, coef_at<S, NP>(a, st.row0, i - 1), st.ring[(j - i) & (NP - 1)]);
-    } else {
-#pragma unroll
-      for (int i = NP; i >= 1; --i)
-        acc = Num<W>::fms(acc, coef_at<S, NP>(a, st.rows[JJ % RC::DR], i - 1), st.ring[(j - i) & (NP - 1)]);
-    }
+    for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, st.cw[i - 1], st.ring[(j - i) & (NP - 1)]);
     st.xs[JJ % RC::DS] = st.xb[t + RC::DS < st.T ? t + RC::DS : st.T - 1];
-    if constexpr (!TI) ld_row_any<S, NP>(st.rows[JJ % RC::DR], st.Ab, st.at, t + RC::DR < st.T ? t + RC::DR : st.T - 1, a.n);
     pin_step();
     st.ring[j] = acc;
     if (t < st.tend) st.yb[t] = convert<S>(acc);
-    fwd_steps<S, NP, TI, JJ + 1>(a, st, tb);
+    fwd_steps_ti<S, NP, JJ + 1>(st, tb);
   }
 }
 
-template <typename S, int NP, bool TI>
-__global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void fwd_rerun_ti(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cid >= (a.rg ? a.G : a.B * a.P)) return;
-  const int64_t b = chunk_signal(a, cid);
-  const Span sp = sig_span(a, b);
-  const int64_t c = cid - sp.cbase;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
   const int64_t t0 = c * a.L;
-  const int64_t rem = sp.T - t0;
+  const int64_t rem = a.T - t0;
   const int len = rem < a.L ? (int)rem : a.L;
-  FwdLane<S, NP> st;
-  st.xb = a.x + sp.xo;
-  st.Ab = a.A + sp.ao;
-  st.yb = a.y + sp.xo;
-  st.at = sp.at;
-  st.T = sp.T;
+  FwdLaneTI<S, NP> st;
+  st.xb = a.x + b * a.T;
+  st.yb = a.y + b * a.T;
+  st.T = a.T;
   st.tend = t0 + len;
 #pragma unroll
-  for (int k = 0; k < RC::DS; ++k) st.xs[k] = opaque_v(st.xb[t0 + k < sp.T ? t0 + k : sp.T - 1]);
-  if constexpr (TI) {
-    ld_row_any<S, NP>(st.row0, st.Ab, 0, 0, a.n);
-  } else {
-#pragma unroll
-    for (int k = 0; k < RC::DR; ++k) {
-      ld_row_any<S, NP>(st.rows[k], st.Ab, st.at, t0 + k < sp.T ? t0 + k : sp.T - 1, a.n);
-#pragma unroll
-      for (int i = 0; i < NP; ++i) st.rows[k][i] = opaque_v(st.rows[k][i]);
-    }
-  }
+  for (int k = 0; k < RC::DS; ++k) st.xs[k] = opaque_v(st.xb[t0 + k < a.T ? t0 + k : a.T - 1]);
+  load_coefs_w<S, NP>(st.cw, a, a.A + b * a.a_bstride);
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = opaque_v(cr[r]);
-  for (int jb = 0; jb < len; jb += RC::K) fwd_steps<S, NP, TI, 0>(a, st, t0 + jb);
+  for (int jb = 0; jb < len; jb += RC::K) fwd_steps_ti<S, NP, 0>(st, t0 + jb);
 }
 
 // y_u of one signal for the adjoint's window: zi for u < 0 (k < n), zero past T,
-// read at an always-valid address and selected afterwards.
+// read at an always-valid address and selected afterwards. Selects, not
+// branches: a load whose address comes out of a branch joins the loop state
+// through a phi, and the loop latch then copies it and waits for it.
 template <typename S>
 __device__ __forceinline__ const S* yw_addr(const S* yb, const S* zib, int n, int64_t T, int64_t u) {
-  // selects, not branches: a load whose address comes out of a branch joins the
-  // loop state through a phi, and the loop latch then copies and waits for it
   const int64_t uc = u < T ? u : T - 1;
   const int64_t k = -u - 1;
   const S* zp = zib + (k < n ? k : n - 1);
@@ -558,43 +530,34 @@ __device__ __forceinline__ bool yw_ok(const S* zib, int n, int64_t T, int64_t u)
   return u >= 0 ? u < T : (zib != nullptr && -u - 1 < n);
 }
 
-// One lane's adjoint state and streams (see bwd_rerun).
+// One lane's adjoint state and streams (see bwd_rerun_ti). DA_PARTIAL also
+// accumulates the lane's share of da.
+enum DaMode { DA_NONE = 0, DA_PARTIAL = 2 };
+
 template <typename S, int NP>
-struct BwdLane {
+struct BwdLaneTI {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
-  W ring[NP];            // pending sums, slot u mod NP for position u
-  S yr[NP];              // conj(y) window: slot u mod NP holds y_u for u in [t-NP, t-1] (exact)
-  W dacc[NP];            // time-invariant da partials
-  S gs[RC::DS];          // g_t in slot (tstart - t) mod DS
-  S ys[RC::DS];          // y_{t-1-NP} (window input of step t), same slots
-  S rows[RC::DR][NP];    // row t in slot (tstart - t) mod DR (time-varying)
-  S row0[NP];            // the row (time-invariant)
+  W ring[NP];    // pending sums, slot u mod NP for position u
+  W cw[NP];      // conj of the coefficients
+  S yr[NP];      // conj(y) window: slot u mod NP holds y_u for u in [t-NP, t-1] (exact)
+  W dacc[NP];    // da partials
+  S gs[RC::DS];  // g_t in slot (tstart - t) mod DS
+  S ys[RC::DS];  // y_{t-1-NP} (window input of step t), same slots
   const S* gb;
-  const S* Ab;
   const S* yb;
   const S* zib;
   S* dxb;
-  S* dAb;
-  int64_t at, T;
-  bool need_da;
+  int64_t T;
 };
-
-template <typename S, int NP>
-__device__ __forceinline__ S yw_val(const BwdLane<S, NP>& st, int n, int64_t u) {
-  return *yw_addr(st.yb, st.zib, n, st.T, u);
-}
-
-// What the adjoint rerun produces besides dx and dzi.
-enum DaMode { DA_NONE = 0, DA_ROWS = 1, DA_PARTIAL = 2 };
 
 // Steps t = tb - JJ for JJ = 0..K-1 (t mod NP = NP-1-JJ mod NP), unrolled by
 // template recursion so that every ring index is a compile-time constant (a
 // runtime index would put the float64 ring in scratch memory). DA is a template
-// argument: a runtime choice between stores and sums inside the steps splits
-// them into branches, and the wait-count pass then drains the loads in flight.
-template <typename S, int NP, bool TI, int DA, int JJ>
-__device__ __forceinline__ void bwd_steps(const ChunkArgs<S>& a, BwdLane<S, NP>& st, int64_t tb) {
+// argument: a runtime choice inside the steps splits them into branches, and the
+// wait-count pass then drains the loads in flight.
+template <typename S, int NP, int DA, int JJ>
+__device__ __forceinline__ void bwd_steps_ti(const ChunkArgs<S>& a, BwdLaneTI<S, NP>& st, int64_t tb) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
   if constexpr (JJ < RC::K) {
@@ -604,97 +567,61 @@ __device__ __forceinline__ void bwd_steps(const ChunkArgs<S>& a, BwdLane<S, NP>&
     const W lam0 = convert<W>(st.gs[JJ % RC::DS]) + st.ring[j];
     const W lam = live ? lam0 : Num<W>::zero();
     st.ring[j] = live ? Num<W>::zero() : st.ring[j];
-    if constexpr (TI) {
 #pragma unroll
-      for (int i = 1; i <= NP; ++i)
-        st.ring[(j - i) & (NP - 1)] =
-            Num<W>::fms(st.ring[(j - i) & (NP - 1)], Num<W>::conj(coef_at<S, NP>(a, st.row0, i - 1)), lam);
-    } else {
-#pragma unroll
-      for (int i = 1; i <= NP; ++i)
-        st.ring[(j - i) & (NP - 1)] =
-            Num<W>::fms(st.ring[(j - i) & (NP - 1)], Num<W>::conj(coef_at<S, NP>(a, st.rows[JJ % RC::DR], i - 1)), lam);
-    }
-    // reloads: the slots just consumed take the step RC::DS (RC::DR) further down
+    for (int i = 1; i <= NP; ++i)
+      st.ring[(j - i) & (NP - 1)] = Num<W>::fms(st.ring[(j - i) & (NP - 1)], st.cw[i - 1], lam);
+    // the slot just consumed takes the step DS further down
     st.gs[JJ % RC::DS] = st.gb[clamp64(t - RC::DS, 0, st.T - 1)];
-    if constexpr (!TI) ld_row_any<S, NP>(st.rows[JJ % RC::DR], st.Ab, st.at, clamp64(t - RC::DR, 0, st.T - 1), a.n);
     if (live && st.dxb) st.dxb[t] = convert<S>(lam);
-    if constexpr (DA != DA_NONE) {
-      if constexpr (DA == DA_ROWS) {
-        if (live) {
-          S* drow = st.dAb + t * a.n;
-          if (a.n == NP) {
+    if constexpr (DA == DA_PARTIAL) {
 #pragma unroll
-            for (int i = 1; i <= NP; ++i)
-              drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
-          } else {
-#pragma unroll
-            for (int i = 1; i <= NP; ++i)
-              if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
-          }
-        }
-      } else {
-#pragma unroll
-        for (int i = 1; i <= NP; ++i)
-          st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
-      }
+      for (int i = 1; i <= NP; ++i)
+        st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
       // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}; its stream
       // slot is consumed here, before it is reloaded
       const bool ynew_ok = yw_ok(st.zib, a.n, st.T, t - 1 - NP);
       st.yr[(j - 1) & (NP - 1)] = ynew_ok ? Num<S>::conj(st.ys[JJ % RC::DS]) : Num<S>::zero();
-      st.ys[JJ % RC::DS] = yw_val(st, a.n, t - RC::DS - 1 - NP);
+      st.ys[JJ % RC::DS] = *yw_addr(st.yb, st.zib, a.n, st.T, t - RC::DS - 1 - NP);
     }
     pin_step();
-    bwd_steps<S, NP, TI, DA, JJ + 1>(a, st, tb);
+    bwd_steps_ti<S, NP, DA, JJ + 1>(a, st, tb);
   }
 }
 
-// Phase 3 (adjoint): one lane per chunk walks its samples backwards from the carry,
-// writes dx and dA (time-varying) or a per-chunk partial of da (time-invariant),
-// and chunk 0 writes dzi. Arithmetic in the wide type, as in the forward rerun.
-template <typename S, int NP, bool TI, int DA>
-__global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
+// Adjoint rerun, time-invariant: one lane per chunk walks its samples backwards
+// from the carry, writes dx and (DA_PARTIAL) the chunk's partial of da; chunk 0
+// writes dzi.
+template <typename S, int NP, int DA>
+__global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cid >= (a.rg ? a.G : a.B * a.P)) return;
-  const int64_t b = chunk_signal(a, cid);
-  const Span sp = sig_span(a, b);
-  const int64_t c = cid - sp.cbase;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
   const int64_t t0 = c * a.L;
-  BwdLane<S, NP> st;
-  st.gb = a.x + sp.xo;  // upstream gradient
-  st.Ab = a.A + sp.ao;
-  st.yb = a.y + sp.xo;
-  st.zib = a.zi ? a.zi + sp.zo : nullptr;
-  st.dxb = a.dx ? a.dx + sp.xo : nullptr;
-  st.dAb = a.dA ? a.dA + sp.dao : nullptr;
-  st.at = sp.at;
-  st.T = sp.T;
-  st.need_da = DA != DA_NONE;
+  BwdLaneTI<S, NP> st;
+  st.gb = a.x + b * a.T;  // upstream gradient
+  st.yb = a.y + b * a.T;
+  st.zib = a.zi ? a.zi + b * a.n : nullptr;
+  st.dxb = a.dx ? a.dx + b * a.T : nullptr;
+  st.T = a.T;
   const int64_t tstart = t0 + a.L - 1;  // first processed (possibly virtual) time
 #pragma unroll
   for (int k = 0; k < RC::DS; ++k) {
     const int64_t t = tstart - k;
-    st.gs[k] = opaque_v(st.gb[clamp64(t, 0, sp.T - 1)]);
-    if constexpr (DA != DA_NONE) st.ys[k] = opaque_v(yw_val(st, a.n, t - 1 - NP));
+    st.gs[k] = opaque_v(st.gb[clamp64(t, 0, a.T - 1)]);
+    if constexpr (DA == DA_PARTIAL) st.ys[k] = opaque_v(*yw_addr(st.yb, st.zib, a.n, a.T, t - 1 - NP));
   }
-  if constexpr (TI) {
-    ld_row_any<S, NP>(st.row0, st.Ab, 0, 0, a.n);
-  } else {
+  load_coefs_w<S, NP>(st.cw, a, a.A + b * a.a_bstride);
 #pragma unroll
-    for (int k = 0; k < RC::DR; ++k) {
-      ld_row_any<S, NP>(st.rows[k], st.Ab, st.at, clamp64(tstart - k, 0, sp.T - 1), a.n);
-#pragma unroll
-      for (int i = 0; i < NP; ++i) st.rows[k][i] = opaque_v(st.rows[k][i]);
-    }
-  }
-  if constexpr (DA != DA_NONE) {
+  for (int i = 0; i < NP; ++i) st.cw[i] = Num<W>::conj(st.cw[i]);
+  if constexpr (DA == DA_PARTIAL) {
 #pragma unroll
     for (int i = 1; i <= NP; ++i) {
       const int64_t u = tstart - i;
-      const S v = yw_val(st, a.n, u);
-      st.yr[(NP - 1 - i + NP) & (NP - 1)] = yw_ok(st.zib, a.n, sp.T, u) ? Num<S>::conj(v) : Num<S>::zero();
+      const S v = *yw_addr(st.yb, st.zib, a.n, a.T, u);
+      st.yr[(NP - 1 - i + NP) & (NP - 1)] = yw_ok(st.zib, a.n, a.T, u) ? Num<S>::conj(v) : Num<S>::zero();
     }
   }
 #pragma unroll
@@ -702,8 +629,8 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = opaque_v(cr[r]);
-  for (int jb = 0; jb < a.L; jb += RC::K) bwd_steps<S, NP, TI, DA, 0>(a, st, tstart - jb);
-  if (DA == DA_PARTIAL) {
+  for (int jb = 0; jb < a.L; jb += RC::K) bwd_steps_ti<S, NP, DA, 0>(a, st, tstart - jb);
+  if constexpr (DA == DA_PARTIAL) {
     W* dp = a.dapart + cid * NP;
 #pragma unroll
     for (int i = 0; i < NP; ++i) dp[i] = a.neg ? -st.dacc[i] : st.dacc[i];
@@ -712,19 +639,22 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
     // pending sums for positions -1-r sit in slots NP-1-r
 #pragma unroll
     for (int r = 0; r < NP; ++r)
-      if (r < a.n) a.dzi[sp.zo + r] = convert<S>(st.ring[NP - 1 - r]);
+      if (r < a.n) a.dzi[b * a.n + r] = convert<S>(st.ring[NP - 1 - r]);
   }
 }
 
-// Reruns of time-varying filters of order above 8 (compiled orders 16 and 32):
-// one lane per chunk, loading each step's row when the step needs it. A row
-// prefetch ring of these orders would not fit the lane's registers.
+// Phase 3 (reruns), general: one lane per chunk, loading each step's row and
+// sample when the step needs it. Time-varying rows do not fit a prefetch ring in
+// the lane's registers at the higher orders, and at the low orders the lane's
+// 4-byte accesses (one L2 request each) bound the kernel whether prefetched or
+// not (measured). Ragged batches and a full dA of time-invariant filters take
+// these kernels too.
 //
 // Forward: one lane per chunk re-filters from its carry and writes y.
 // The recurrence runs in the wide type (float64 for float32 data), so the only
 // single-precision rounding of an output is its final store.
 template <typename S, int NP>
-__global__ __launch_bounds__(256) void fwd_rerun_rows(ChunkArgs<S> a) {
+__global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cid >= (a.rg ? a.G : a.B * a.P)) return;
@@ -761,7 +691,7 @@ __global__ __launch_bounds__(256) void fwd_rerun_rows(ChunkArgs<S> a) {
 
 // y_u of one signal for the adjoint's window: zi for u < 0 (k < n), zero past T.
 template <typename S>
-__device__ __forceinline__ S yval_at_rows(const S* yb, const S* zib, int n, int64_t T, int64_t u) {
+__device__ __forceinline__ S yval_at(const S* yb, const S* zib, int n, int64_t T, int64_t u) {
   if (u >= 0) return (u < T) ? yb[u] : Num<S>::zero();
   const int64_t k = -u - 1;
   return (zib && k < n) ? zib[k] : Num<S>::zero();
@@ -769,7 +699,7 @@ __device__ __forceinline__ S yval_at_rows(const S* yb, const S* zib, int n, int6
 
 // One lane's adjoint state and streams (see bwd_rerun).
 template <typename S, int NP>
-struct BwdRowLane {
+struct BwdLane {
   using W = typename Wide<S>::type;
   W ring[NP];  // pending sums, slot u mod NP for position u
   S yr[NP];    // conj(y) window: slot u mod NP holds y_u for u in [t-NP, t-1] (data type, exact)
@@ -788,7 +718,7 @@ struct BwdRowLane {
 // recursion so that every ring index is a compile-time constant (a runtime index
 // would put the float64 ring in scratch memory).
 template <typename S, int NP, int JJ>
-__device__ __forceinline__ void bwd_row_steps(const ChunkArgs<S>& a, BwdRowLane<S, NP>& st, int64_t tb) {
+__device__ __forceinline__ void bwd_steps(const ChunkArgs<S>& a, BwdLane<S, NP>& st, int64_t tb) {
   using W = typename Wide<S>::type;
   if constexpr (JJ < NP) {
     constexpr int j = NP - 1 - JJ;
@@ -818,9 +748,9 @@ __device__ __forceinline__ void bwd_row_steps(const ChunkArgs<S>& a, BwdRowLane<
     }
     if (st.need_da) {
       // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
-      st.yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval_at_rows(st.yb, st.zib, a.n, st.T, t - 1 - NP));
+      st.yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval_at(st.yb, st.zib, a.n, st.T, t - 1 - NP));
     }
-    bwd_row_steps<S, NP, JJ + 1>(a, st, tb);
+    bwd_steps<S, NP, JJ + 1>(a, st, tb);
   }
 }
 
@@ -828,7 +758,7 @@ __device__ __forceinline__ void bwd_row_steps(const ChunkArgs<S>& a, BwdRowLane<
 // writes dx and dA (time-varying) or a per-chunk partial of da (time-invariant),
 // and chunk 0 writes dzi. Arithmetic in the wide type, as in the forward rerun.
 template <typename S, int NP>
-__global__ __launch_bounds__(256) void bwd_rerun_rows(ChunkArgs<S> a) {
+__global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cid >= (a.rg ? a.G : a.B * a.P)) return;
@@ -836,7 +766,7 @@ __global__ __launch_bounds__(256) void bwd_rerun_rows(ChunkArgs<S> a) {
   const Span sp = sig_span(a, b);
   const int64_t c = cid - sp.cbase;
   const int64_t t0 = c * a.L;
-  BwdRowLane<S, NP> st;
+  BwdLane<S, NP> st;
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = cr[r];
@@ -853,11 +783,11 @@ __global__ __launch_bounds__(256) void bwd_rerun_rows(ChunkArgs<S> a) {
   if (st.need_da) {
 #pragma unroll
     for (int i = 1; i <= NP; ++i)
-      st.yr[(NP - 1 - i + NP) & (NP - 1)] = Num<S>::conj(yval_at_rows(st.yb, st.zib, a.n, st.T, tstart - i));
+      st.yr[(NP - 1 - i + NP) & (NP - 1)] = Num<S>::conj(yval_at(st.yb, st.zib, a.n, st.T, tstart - i));
   }
 #pragma unroll
   for (int i = 0; i < NP; ++i) st.dacc[i] = Num<W>::zero();
-  for (int jb = 0; jb < a.L; jb += NP) bwd_row_steps<S, NP, 0>(a, st, tstart - jb);
+  for (int jb = 0; jb < a.L; jb += NP) bwd_steps<S, NP, 0>(a, st, tstart - jb);
   if (a.dapart) {
     W* dp = a.dapart + cid * NP;
 #pragma unroll
@@ -1033,7 +963,7 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
     if (nz > 0) hipLaunchKernelGGL((ti_zsum<S, NP, true>), dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, a);
     hipLaunchKernelGGL((carry_ti<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
-    hipLaunchKernelGGL((fwd_rerun<S, NP, true>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((fwd_rerun_ti<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   if (a.P > 1) {
@@ -1050,10 +980,7 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_serial<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
   const int64_t nch = chunk_count(a);
-  if constexpr (NP <= 8)
-    hipLaunchKernelGGL((fwd_rerun<S, NP, false>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((fwd_rerun_rows<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((fwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
@@ -1066,18 +993,14 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     hipLaunchKernelGGL((carry_ti<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
     const dim3 g((unsigned)((nch + 255) / 256));
-    // a full per-sample dA of a time-invariant filter takes the time-varying
+    // a full per-sample dA of a time-invariant filter takes the per-step row
     // kernel, which reads the one row at stride 0
     if (a.dapart)
-      hipLaunchKernelGGL((bwd_rerun<S, NP, true, DA_PARTIAL>), g, dim3(256), 0, st, a);
-    else if (a.dA) {
-      if constexpr (NP <= 8)
-        hipLaunchKernelGGL((bwd_rerun<S, NP, false, DA_ROWS>), g, dim3(256), 0, st, a);
-      else
-        hipLaunchKernelGGL((bwd_rerun_rows<S, NP>), g, dim3(256), 0, st, a);
-    }
+      hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL>), g, dim3(256), 0, st, a);
+    else if (a.dA)
+      hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(256), 0, st, a);
     else
-      hipLaunchKernelGGL((bwd_rerun<S, NP, true, DA_NONE>), g, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE>), g, dim3(256), 0, st, a);
     if (a.dapart) {
       const int64_t nd = a.B * a.n;
       hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);
@@ -1098,15 +1021,7 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_serial<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
   const int64_t nch = chunk_count(a);
-  const dim3 g((unsigned)((nch + 255) / 256));
-  if constexpr (NP <= 8) {
-    if (a.dA)
-      hipLaunchKernelGGL((bwd_rerun<S, NP, false, DA_ROWS>), g, dim3(256), 0, st, a);
-    else
-      hipLaunchKernelGGL((bwd_rerun<S, NP, false, DA_NONE>), g, dim3(256), 0, st, a);
-  } else {
-    hipLaunchKernelGGL((bwd_rerun_rows<S, NP>), g, dim3(256), 0, st, a);
-  }
+  hipLaunchKernelGGL((bwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
   if (a.dapart) {
     const int64_t nd = a.B * a.n;
     hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);
