@@ -236,11 +236,18 @@ def bench_workload(name, steps, warmup):
         y = torch.empty_like(x)
         dx, dA, dzi = torch.empty_like(x), torch.empty_like(A), torch.empty_like(zi)
 
+        def fwd(grp):
+            n, table, Kg, G, maxp, L = grp
+            ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L)
+
+        def bwd(grp):
+            n, table, Kg, G, maxp, L = grp
+            ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True)
+
         def step():
-            for n, table, Kg, G, maxp, L in plan.groups:
-                ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L)
-            for n, table, Kg, G, maxp, L in plan.groups:
-                ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True)
+            # the order groups run on the plan's side streams (graph branches)
+            plan.run(fwd)
+            plan.run(bwd)
 
         S = sum(lengths)
         alg = sum((3 * N + 5) * 4 * T for T, N in zip(lengths, orders))

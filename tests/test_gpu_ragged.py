@@ -95,8 +95,12 @@ def test_ragged_full_workload_and_graph_replay():
     def step():
         y = RaggedLPC.forward(X, AA, Z, plan)
         dx, da, dzi = torch.empty_like(X), torch.empty_like(AA), torch.empty_like(Z)
-        for n, table, K, G, maxp, L in plan.groups:
+
+        def bwd(grp):
+            n, table, K, G, maxp, L = grp
             ops.lpc_ragged_backward_(Gd, AA, Z, y, table, dx, da, dzi, K, G, maxp, n, L, True, True, True)
+
+        plan.run(bwd)  # order groups on the plan's side streams
         return y, dx, da, dzi
 
     eager = step()

@@ -44,14 +44,14 @@ inline size_t elem_size(int dtype) {
 }
 
 // Chunk length: a multiple of the staging block, sized so that the rerun phase
-// has about 32K lanes in flight, capped at 4096 samples. Single-precision data
-// keeps chunks at least 16*NP long: every chunk boundary adds the rounding of
-// an NP x NP carry product, and over longer chunks the basis responses have
-// decayed, which keeps the result at the accuracy of a serial recurrence.
-inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool single) {
+// has about 32K lanes in flight, capped at 4096 samples, and at least 4*NP long
+// so that the carry chains stay short. Summaries and carries are in the wide
+// type (float64 for single-precision data), so a chunk boundary adds no
+// single-precision rounding and shorter chunks cost no accuracy.
+inline int pick_chunk(int64_t B, int64_t T, int np, int requested) {
   const int blk = chunk_block(np);
   int64_t L = requested > 0 ? requested : (B * T + 32767) / 32768;
-  if (requested <= 0 && single && L < 16 * np) L = 16 * np;
+  if (requested <= 0 && L < 4 * np) L = 4 * np;
   if (L < blk) L = blk;
   if (L > 4096 && requested <= 0) L = 4096;
   if (L > (int64_t)1 << 24) L = (int64_t)1 << 24;
@@ -81,7 +81,7 @@ inline int device_cus() {
 // are chosen first, P = 64 * CUs / B, and the length rounded up to 16 samples
 // from there (which can only lower P). At least 16 * 16 samples.
 inline int pick_chunk16(int64_t B, int64_t T, int requested) {
-  if (requested > 0) return pick_chunk(B, T, 16, requested, true);
+  if (requested > 0) return pick_chunk(B, T, 16, requested);
   int64_t P = 64 * (int64_t)device_cus() / (B > 0 ? B : 1);
   if (P < 1) P = 1;
   int64_t L = (T + P - 1) / P;
@@ -161,7 +161,7 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   else if (chunk16_ok(dtype, n, flags, neg, 256, T))
     p.L = pick_chunk16(B, T, chunk_len);
   else
-    p.L = pick_chunk(B, T, p.np, chunk_len, dtype == TLPC_F32 || dtype == TLPC_C64);
+    p.L = pick_chunk(B, T, p.np, chunk_len);
   p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
   const size_t ws_es = wide_size(dtype);
   // time-invariant summaries are the zero-state vectors only (Phi is per signal)
@@ -455,8 +455,8 @@ TLPC_EXPORT int tlpc_lpc_ragged_plan(int64_t K, const int64_t* x_off, const int6
     if (lengths[k] < 1) return TLPC_EINVAL;
     total += lengths[k];
   }
-  const bool single = dtype == TLPC_F32 || dtype == TLPC_C64;
-  const int L = pick_chunk(K > 0 ? K : 1, K > 0 ? (total + K - 1) / K : 1, np, chunk_len, single);
+  if (elem_size(dtype) == 0) return TLPC_EDTYPE;
+  const int L = pick_chunk(K > 0 ? K : 1, K > 0 ? (total + K - 1) / K : 1, np, chunk_len);
   int64_t G = 0, maxP = 0;
   for (int64_t k = 0; k < K; ++k) {
     const int64_t P = (lengths[k] + L - 1) / L;

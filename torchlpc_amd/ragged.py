@@ -12,7 +12,10 @@ holds the metadata (lengths, orders, offsets, and per order a device table that
 maps chunks to signals), built once on the host; the filter then runs as one
 set of chunked-scan launches per order present (summaries, carries, reruns),
 forward and fused backward, so a whole ragged step can be captured in one HIP
-graph.
+graph. The order groups write disjoint parts of the outputs, so they run on up
+to four side streams forked from and joined back into the caller's stream (a
+process gets four hardware queues); in a captured graph they become parallel
+branches.
 """
 from __future__ import annotations
 
@@ -66,6 +69,38 @@ class RaggedPlan:
                 chunk_len,
             )
             self.groups.append((n, table.to(self.device), int(idx.numel()), G, maxp, L))
+        # streams for the groups: longest first onto the least loaded stream,
+        # a group's cost taken as its padded order times its samples
+        self._streams: Optional[List[torch.cuda.Stream]] = None
+        nstreams = min(len(self.groups), 4) if self.device.type == "cuda" else 1
+        load = [0] * nstreams
+        self.lanes: List[int] = [0] * len(self.groups)
+        cost = []
+        for gi, (n, _, _, _, _, _) in enumerate(self.groups):
+            np_ = 1 << (n - 1).bit_length()
+            cost.append((np_ * int(lengths_t[orders_t == n].sum()), gi))
+        for c, gi in sorted(cost, reverse=True):
+            s = min(range(nstreams), key=lambda i: load[i])
+            self.lanes[gi] = s
+            load[s] += c
+
+    def run(self, fn) -> None:
+        """Calls ``fn(group)`` for every order group, on the plan's side streams
+        when there is more than one."""
+        if self.device.type != "cuda" or len(self.groups) < 2:
+            for grp in self.groups:
+                fn(grp)
+            return
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream(device=self.device) for _ in range(max(self.lanes) + 1)]
+        cur = torch.cuda.current_stream(self.device)
+        for st in self._streams:
+            st.wait_stream(cur)
+        for gi, grp in enumerate(self.groups):
+            with torch.cuda.stream(self._streams[self.lanes[gi]]):
+                fn(grp)
+        for st in self._streams:
+            cur.wait_stream(st)
 
     def split(self, packed: torch.Tensor, kind: str = "x") -> List[torch.Tensor]:
         """Views of the K signals of a packed tensor (kind "x", "a" or "zi")."""
@@ -85,8 +120,12 @@ class RaggedLPC(Function):
     @staticmethod
     def forward(x: torch.Tensor, a: torch.Tensor, zi: torch.Tensor, plan: RaggedPlan) -> torch.Tensor:
         y = torch.empty_like(x)
-        for n, table, K, G, maxp, L in plan.groups:
+
+        def one(grp):
+            n, table, K, G, maxp, L = grp
             ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L)
+
+        plan.run(one)
         return y
 
     @staticmethod
@@ -105,10 +144,14 @@ class RaggedLPC(Function):
         da = torch.empty_like(a) if need_a else a.new_empty(0)
         dzi = torch.empty_like(zi) if need_zi else zi.new_empty(0)
         if need_x or need_a or need_zi:
-            for n, table, K, G, maxp, L in ctx.plan.groups:
+
+            def one(grp):
+                n, table, K, G, maxp, L = grp
                 ops().lpc_ragged_backward_(
                     g, a, zi, y, table, dx, da, dzi, K, G, maxp, n, L, need_x, need_a, need_zi
                 )
+
+            ctx.plan.run(one)
         return (dx if need_x else None, da if need_a else None, dzi if need_zi else None, None)
 
 
