@@ -256,6 +256,115 @@ __global__ __launch_bounds__(64) void carry_serial(ChunkArgs<S> a) {
   }
 }
 
+// Pipelined carry: one 256-thread workgroup per signal. Wave 0 walks the chain;
+// waves 1-3 take turns staging batches of CB links from the summaries into
+// registers DEPTH batches ahead and committing them into an LDS double buffer
+// (matrix rows padded to NP+1 elements), one LDS barrier per batch. A link is
+// then LDS-bound only: the chain lane r reads row r of M and the broadcast state,
+// and the next link's row is read before the current link's products.
+template <typename S, int NP>
+struct PipeCfg {
+  using W = typename Wide<S>::type;
+  static constexpr int ROWP = NP + 1;
+  static constexpr int LINK = NP * ROWP + NP;  // staged elements per link
+  static constexpr int SUMM = NP * NP + NP;    // summary elements per link
+  static constexpr int BUDGET = 72 * 1024;
+  static constexpr int CB = (2 * 8 * LINK * (int)sizeof(W) <= BUDGET) ? 8
+                            : ((2 * 4 * LINK * (int)sizeof(W) <= BUDGET) ? 4 : 2);
+  static constexpr int PER = (CB * SUMM + 63) / 64;  // elements per loader lane per batch
+  static constexpr int DEPTH = 3;
+};
+
+template <typename S, int NP, bool FWD>
+__global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
+  using C = PipeCfg<S, NP>;
+  __shared__ W buf[2][C::CB][C::LINK];
+  __shared__ W sv[NP];
+  const int64_t b = blockIdx.x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const Span sp = sig_span(a, b);
+  const int64_t P = sp.P;
+  const int64_t links = P - 1;
+  const int64_t nbatch = (links + C::CB - 1) / C::CB;
+  W* carry = a.carry + sp.cbase * NP;
+  W stage[C::PER];
+  // link l (in processing order) is chunk FWD ? l : P-1-l
+  // loads are unconditional (indices clamped to the last link: the chain never
+  // reads those slots), so that nothing waits for them before the commit
+  auto fetch = [&](int64_t j) {
+    const int64_t l0 = j * C::CB;
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) {
+      const int e = min(q * 64 + lane, C::CB * C::SUMM - 1);
+      const int li = e / C::SUMM, off = e - li * C::SUMM;
+      const int64_t l = min(l0 + li, links - 1);
+      const int64_t c = FWD ? l : P - 1 - l;
+      stage[q] = a.summ[summ_slot(a, b, sp, c, FWD) * C::SUMM + off];
+    }
+  };
+  auto commit = [&](int64_t j) {
+    W(*dst)[C::LINK] = buf[j & 1];
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) {
+      const int e = q * 64 + lane;
+      const int li = e / C::SUMM, off = e - li * C::SUMM;
+      if (e < C::CB * C::SUMM) dst[li][off < NP * NP ? off + off / NP : off + NP] = stage[q];
+    }
+  };
+  const bool own = wave == 0 && lane < NP;
+  W s = Num<W>::zero();
+  if (FWD && own && lane < a.n && a.zi) s = convert<W>(a.zi[sp.zo + lane]);
+  const int row = lane < NP ? lane : NP - 1;
+  for (int64_t i = -C::DEPTH; i < nbatch; ++i) {
+    if (wave > 0) {
+      const int64_t jn = i + C::DEPTH;  // issue
+      if (jn < nbatch && 1 + jn % 3 == wave) fetch(jn);
+      const int64_t jc = i + 1;  // commit (fetched DEPTH - 1 iterations ago)
+      if (jc >= 0 && jc < nbatch && 1 + jc % 3 == wave) commit(jc);
+    } else if (i >= 0) {
+      const int64_t l0 = i * C::CB;
+      const int nb = (int)((links - l0) < C::CB ? (links - l0) : C::CB);
+      const W(*src)[C::LINK] = buf[i & 1];
+      W m[2][NP];
+      W zv[2];
+      auto ldm = [&](int li, W(&mm)[NP], W& z) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) mm[k] = src[li][row * C::ROWP + k];
+        z = src[li][NP * C::ROWP + row];
+      };
+      ldm(0, m[0], zv[0]);
+#pragma unroll
+      for (int li = 0; li < C::CB; ++li) {
+        if (li + 1 < C::CB) ldm(li + 1, m[(li + 1) & 1], zv[(li + 1) & 1]);
+        if (li < nb) {
+          const int64_t l = l0 + li;
+          if (own) {
+            carry[(FWD ? l : P - 1 - l) * NP + lane] = s;
+            sv[lane] = s;
+          }
+          // the state reaches every lane through LDS (one wave: in order)
+          W acc0 = zv[li & 1], acc1 = Num<W>::zero(), acc2 = Num<W>::zero(), acc3 = Num<W>::zero();
+#pragma unroll
+          for (int k = 0; k < NP; k += 4) {
+            acc0 = Num<W>::fma(acc0, m[li & 1][k], sv[k]);
+            if (k + 1 < NP) acc1 = Num<W>::fma(acc1, m[li & 1][k + 1], sv[k + 1]);
+            if (k + 2 < NP) acc2 = Num<W>::fma(acc2, m[li & 1][k + 2], sv[k + 2]);
+            if (k + 3 < NP) acc3 = Num<W>::fma(acc3, m[li & 1][k + 3], sv[k + 3]);
+          }
+          s = (acc0 + acc1) + (acc2 + acc3);
+        }
+      }
+    }
+    // LDS-only barrier: the loaders' global loads stay in flight across it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  }
+  if (own) carry[(FWD ? P - 1 : 0) * NP + lane] = s;
+}
+
 // Blocked carry scan for small NP: one workgroup per signal; each thread composes
 // the affine maps of a contiguous run of chunks, the runs are combined by a
 // workgroup scan, then each thread re-walks its run from its carry.
@@ -978,7 +1087,7 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
     }
   }
   if (!blocked)
-    hipLaunchKernelGGL((carry_serial<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((carry_pipe<S, NP, true>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
   hipLaunchKernelGGL((fwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
@@ -1019,7 +1128,7 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     }
   }
   if (!blocked)
-    hipLaunchKernelGGL((carry_serial<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((carry_pipe<S, NP, false>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
   hipLaunchKernelGGL((bwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
   if (a.dapart) {
