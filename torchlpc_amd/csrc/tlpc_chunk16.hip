@@ -606,22 +606,27 @@ __global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
   double* carry = a.carry + b * P * NP;
   constexpr int PER = CB * SUMM / 64;  // doubles per lane per batch
   double stage[PER];
+  // Loads are unconditional, at a link index clamped into the chain (the chain
+  // never reads the slots past its end), with the adjoint's choices made on the
+  // address; a load under a branch would be waited for at once.
   auto fetch = [&](int64_t j) {
     const int64_t l0 = j * CB;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int e = q * 64 + lane;
       const int li = e / SUMM, off = e - li * SUMM;
-      const int64_t l = l0 + li;
-      double v = 0.0;
-      if (l < links) {
-        if (FWD) {
-          v = summ[l * SUMM + off];
-        } else if (off >= NP * NP) {
-          v = wsum[(links - 1 - l) * NP + off - NP * NP];
-        } else if (l > 0) {
-          v = summ[(links - l) * SUMM + off];
-        }
+      const int64_t l = min(l0 + li, links - 1);
+      double v;
+      if constexpr (FWD) {
+        v = summ[l * SUMM + off];
+      } else {
+        // link l is chunk P-1-l: its w, and Phi^T of the forward summary of that
+        // chunk (link 0, the last chunk, meets the zero state and has no matrix)
+        const bool w = off >= NP * NP;
+        const double* p = w ? wsum + (links - 1 - l) * NP + (off - NP * NP)
+                            : summ + (links - (l > 0 ? l : 1)) * SUMM + off;
+        const double raw = *p;
+        v = (w || l > 0) ? raw : 0.0;
       }
       stage[q] = v;
     }
