@@ -227,35 +227,10 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
 }
 
 // ----------------------------------------------------------------------------
-// Phase 2: carries. One wavefront per signal; lane r owns state component r.
+// Phase 2: carries, per signal.
 // Forward: carry[c] = state entering chunk c, carry[0] = zi, carry[c+1] = Phi_c carry[c] + z_c.
 // Adjoint: carry[c] = pending sums entering chunk c from the right,
 //          carry[P-1] = 0, carry[c-1] = Psi_c carry[c] + w_c (summary of chunk c at c-1).
-template <typename S, int NP, bool FWD>
-__global__ __launch_bounds__(64) void carry_serial(ChunkArgs<S> a) {
-  const int64_t b = blockIdx.x;
-  const int r = threadIdx.x;
-  const Span sp = sig_span(a, b);
-  const int64_t P = sp.P;
-  using W = typename Wide<S>::type;
-  W s = Num<W>::zero();
-  if (FWD && r < a.n && a.zi) s = convert<W>(a.zi[sp.zo + r]);
-  W* carry = a.carry + sp.cbase * NP;
-  for (int64_t step = 0; step < P; ++step) {
-    const int64_t c = FWD ? step : P - 1 - step;
-    if (r < NP) carry[c * NP + r] = s;
-    if (step == P - 1) break;
-    const W* sm = a.summ + summ_slot(a, b, sp, c, FWD) * ChunkCfg<S, NP>::SUMM;
-    W acc = (r < NP) ? sm[NP * NP + r] : Num<W>::zero();
-#pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      const W sk = shfl(s, k);
-      if (r < NP) acc = Num<W>::fma(acc, sm[r * NP + k], sk);
-    }
-    s = acc;
-  }
-}
-
 // Pipelined carry: one 256-thread workgroup per signal. Wave 0 walks the chain;
 // waves 1-3 take turns staging batches of CB links from the summaries into
 // registers DEPTH batches ahead and committing them into an LDS double buffer
