@@ -1,15 +1,17 @@
 // Chunked scan for order 16, float32 data (the B=256, T=48000, N=16 shape and
 // any other float32 order-16 batch), built from the DPP blocks of tlpc_dpp16.h:
 //
-//   summary  forward: 16 chunks per workgroup, waves 0-3 are 16 matrix rows (one
-//            chunk per 16-lane row, float64 basis responses -> Phi), wave 4 is 16
-//            quads (float64 zero-state response -> z); chunks 0..P-2.
+//   summary  forward: 16 chunks per workgroup, one 16-lane DPP row per chunk:
+//            lane r sweeps the chunk backwards with the transposed recurrence
+//            from a unit at output r, which gives row r of Phi (the pending
+//            sums left before the chunk) and z[r] (the lambdas met on the way,
+//            dotted with x); chunks 0..P-2.
 //            adjoint: quads only (zero-state response of the transposed
 //            recurrence -> w), chunks 1..P-1. The adjoint transition matrix of a
 //            chunk is Phi^T: the transposed recurrence is the adjoint of the
 //            forward one, so its homogeneous chunk map is the transpose. An
 //            adjoint that is handed the forward's summaries skips Phi entirely;
-//            otherwise it computes Phi with the forward's matrix rows.
+//            otherwise it computes Phi with the summary kernel's rows.
 //   carry    one wave per signal, float64.
 //   rerun    one quad per chunk from its float64 carry, float64 arithmetic on the
 //            float32 data (the rerun produces the outputs, so it keeps their
@@ -28,7 +30,6 @@ namespace c16 {
 
 using namespace dpp16;
 
-constexpr int SUM_THREADS = 320;  // 4 matrix waves + 1 quad wave = 16 chunks
 constexpr int RERUN_THREADS = 256;
 
 // ---------------------------------------------------------------------------
@@ -118,24 +119,6 @@ __device__ __forceinline__ void store16(float* vb, int tb, int T, int ql, float4
     if (u + 1 < T) p[1] = o.y;
     if (u + 2 < T) p[2] = o.z;
     if (u + 3 < T) p[3] = o.w;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Matrix rows. Lane kk of a 16-lane row reads A[t, kk+1] (ap = &A[t0, kk]);
-// cf is a 16-slot ring: step J of a block consumes slot J and then reloads it
-// for step J of the next block, so nothing a block loads is used in the same
-// block (the order in which the compiler issues the loads does not matter).
-// The rows run whole chunks (t0 + L <= T), so the next block lies inside the
-// signal whenever it is used; its base np is clamped once per block to keep the
-// last prefetch in bounds, and the step loads are immediate offsets from it.
-template <int J>
-__device__ __forceinline__ void fwd_row_block(double (&r)[NP], float (&cf)[NP], const float* np) {
-  if constexpr (J < NP) {
-    r[J] = fwd_row_step<J>((double)cf[J], r);
-    cf[J] = np[J * NP];
-    pin_loads();
-    fwd_row_block<J + 1>(r, cf, np);
   }
 }
 
@@ -433,48 +416,68 @@ __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const 
 }
 
 // ---------------------------------------------------------------------------
-// Forward summaries: chunks c = 0..P-2 (full length L).
-template <bool WITH_Z, bool VEC>
-__global__ __launch_bounds__(SUM_THREADS, WITH_Z ? (VEC ? 5 : 4) : 5) void sum_fwd(ChunkArgs<float> a) {
+// Forward summaries: chunks c = 0..P-2 (full length L), 16 per workgroup, one
+// 16-lane DPP row per chunk.
+//
+// Lane r of a row computes row r of Phi and z[r] together, by sweeping the
+// chunk backwards with the transposed recurrence from a unit at output r (the
+// sample t0+L-1-r): the pending sums left for the positions before the chunk are
+// d y_{t0+L-1-r} / d y_{t0-1-k} = Phi[r][k], and the lambdas met on the way are
+// d y_{t0+L-1-r} / d x_t, so z[r] = sum_t lambda_t x_t. One pass over A serves
+// both (the forward form needs a 17th, zero-state, recurrence beside the 16
+// basis responses), each step's 16 products are independent of each other, and
+// the sweep ends on the chunk's first rows, which the forward rerun reads first.
+//
+// Lane kk holds A[t, kk+1] of the step (a 16-slot ring, consumed then reloaded
+// for the block below: the row base np is clamped once per block so the last
+// prefetch stays in the signal) and x of step tb+kk of the block.
+template <int J, bool WITH_Z>
+__device__ __forceinline__ void adj_row_block(double (&p)[NP], double& z, float (&cf)[NP], double xv,
+                                              const float* np) {
+  if constexpr (J >= 0) {
+    const double lam = adj_row_step<J>(p, (double)cf[J]);
+    if constexpr (WITH_Z) fma_bcast<J>(z, xv, lam);
+    cf[J] = np[J * NP];
+    pin_loads();
+    adj_row_block<J - 1, WITH_Z>(p, z, cf, xv, np);
+  }
+}
+
+template <bool WITH_Z>
+__global__ __launch_bounds__(256) void sum_fwd(ChunkArgs<float> a) {
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t cps = a.P - 1;
   const int64_t nch = a.B * cps;
+  const int64_t cid = (int64_t)blockIdx.x * 16 + wave * 4 + (lane >> 4);
+  if (cid >= nch) return;
   const int L = a.L;
-  const int T = (int)a.T;
-  if (wave < 4) {
-    const int64_t cid = (int64_t)blockIdx.x * 16 + wave * 4 + (lane >> 4);
-    if (cid >= nch) return;
-    const int kk = lane & 15;
-    const int64_t b = cid / cps;
-    const int c = (int)(cid - b * cps);
-    const int t0 = c * L;
-    const float* ap = a.A + (b * a.T + t0) * NP + kk;
-    const int jlim = T - 1 - t0;
-    double r[NP];
+  const int kk = lane & 15;
+  const int64_t b = cid / cps;
+  const int c = (int)(cid - b * cps);
+  const int t0 = c * L;
+  const float* ap = a.A + (b * a.T + t0) * NP + kk;
+  const float* xp = a.x + b * a.T + t0 + kk;
+  double p[NP];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) r[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // y_{-1-kk} = 1
-    float cf[NP];
+  for (int i = 0; i < NP; ++i) p[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // unit at t0+L-1-kk
+  double z = 0.0;
+  float cf[NP];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) cf[j] = opaque(ap[min(j, jlim) * NP]);
-    // jlim >= L - 1 >= NP - 1: the clamped base keeps all NP rows in the signal
-    for (int jb = 0; jb < L; jb += NP) fwd_row_block<0>(r, cf, ap + min(jb + NP, jlim - (NP - 1)) * NP);
-    double* out = a.summ + cid * (NP * NP + NP);
-#pragma unroll
-    for (int rr = 0; rr < NP; ++rr) out[rr * NP + kk] = r[NP - 1 - rr];
-  } else if (WITH_Z) {
-    const int64_t cid = (int64_t)blockIdx.x * 16 + (lane >> 2);
-    if (cid >= nch) return;
-    const int ql = lane & 3;
-    const int64_t b = cid / cps;
-    const int c = (int)(cid - b * cps);
-    double R[4] = {0.0, 0.0, 0.0, 0.0};
-    const FwdIo io{a.A + b * a.T * NP, a.x + b * a.T, nullptr, T};
-    fwd_quad_run<false, VEC, true>(R, io, c * L, L, ql);  // full chunks: whole blocks
-    double* out = a.summ + cid * (NP * NP + NP) + NP * NP;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) out[4 * ql + m] = R[3 - m];  // z[4ql+m] = y_{L-1-4ql-m}
+  for (int j = 0; j < NP; ++j) cf[j] = opaque(ap[(L - NP + j) * NP]);
+  float xf = WITH_Z ? opaque(xp[L - NP]) : 0.f;
+  for (int jb = L - NP; jb >= 0; jb -= NP) {
+    const int jn = jb >= NP ? jb - NP : 0;  // the block below (block 0 again after the last)
+    const double xv = (double)xf;
+    float xn = 0.f;
+    if constexpr (WITH_Z) xn = xp[jn];
+    adj_row_block<NP - 1, WITH_Z>(p, z, cf, xv, ap + jn * NP);
+    xf = xn;
   }
+  double* out = a.summ + cid * (NP * NP + NP);
+#pragma unroll
+  for (int k = 0; k < NP; ++k) out[kk * NP + k] = p[NP - 1 - k];  // Phi[kk][k]
+  if constexpr (WITH_Z) out[NP * NP + kk] = z;
 }
 
 // Adjoint zero-state responses: chunks c = 1..P-1, one quad each (the last may
@@ -704,13 +707,10 @@ static int stream_mode(const ChunkArgs<float>& a) {
 int launch_chunk16_fwd_summary(ChunkArgs<float> a, bool with_z, hipStream_t st) {
   const int64_t n = a.B * (a.P - 1);
   const dim3 grid((unsigned)((n + 15) / 16));
-  const bool v = vec_ok(a);
-  if (n > 0 && with_z && v)
-    hipLaunchKernelGGL((c16::sum_fwd<true, true>), grid, dim3(c16::SUM_THREADS), 0, st, a);
-  else if (n > 0 && with_z)
-    hipLaunchKernelGGL((c16::sum_fwd<true, false>), grid, dim3(c16::SUM_THREADS), 0, st, a);
+  if (n > 0 && with_z)
+    hipLaunchKernelGGL(c16::sum_fwd<true>, grid, dim3(256), 0, st, a);
   else if (n > 0)
-    hipLaunchKernelGGL((c16::sum_fwd<false, false>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(c16::sum_fwd<false>, grid, dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 int launch_chunk16_adj_summary(ChunkArgs<float> a, hipStream_t st) {

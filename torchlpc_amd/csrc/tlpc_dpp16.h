@@ -2,9 +2,9 @@
 // the per-signal kernel and the order-16 chunked-scan kernels.
 //
 // Matrix rows: a 16-lane DPP row advances 16 independent float64 recurrences
-// (one basis vector per lane) through one sub-chunk. Lane i of the row holds
-// coefficient A[t, i+1] of the current step and every lane reads it through a
-// row_newbcast operand of v_fmac_f64: one instruction per lag, no LDS traffic.
+// (one per lane) through a chunk. Lane i of the row holds coefficient A[t, i+1]
+// of the current step and every lane reads it through a row_newbcast operand of
+// v_fmac_f64: one instruction per lag, no LDS traffic.
 //
 // Quads: four lanes share one serial recurrence; lane ql owns lags 4ql+1..4ql+4
 // (a float4 of the A row per step, so a quad loads a whole 64-byte row) and a
@@ -20,40 +20,51 @@ constexpr int NP = 16;
 constexpr int PF = 16;  // rows in flight per quad lane (one 16-step block)
 
 // ---------------------------------------------------------------------------
-// Forward matrix step (slot J = t mod 16 of the output ring r, r[u mod 16] = y_u):
-//   returns -sum_{i=1..16} A[t,i] * r[(J - i) & 15]
-// One accumulator chain, oldest lag first: the newest value r[J-1], which the
-// previous step produced, is read last, and no closing add is needed (two chains
-// would cost a second zeroed accumulator and an add per step; the waves sharing a
-// SIMD hide the longer chain). One asm statement, opened by s_nop 1: the two wait
-// states a DPP source needs after the VALU conversion that produced it.
+// Adjoint matrix step (slot J = t mod 16 of the pending-sum window p,
+// p[u mod 16] = pending sum of position u), the transposed recurrence:
+//   lambda = p[J];  p[(J - i) & 15] -= A[t,i] * lambda   (i = 1..16)
+// where i = 16 lands on slot J itself, which now stands for position t-16 and
+// starts from zero (`fresh`). Returns lambda. The 16 products are independent of
+// each other: the next step (slot J-1) waits only for the first. One asm
+// statement, opened by s_nop 1 for the conversion that produced `a`.
 template <int J>
-__device__ __forceinline__ double fwd_row_step(double a, const double (&r)[NP]) {
-  double acc = 0.0;
+__device__ __forceinline__ double adj_row_step(double (&p)[NP], double a) {
+  const double lam = p[J];
+  double fresh = 0.0;
   asm("s_nop 1\n\t"
-      "v_fmac_f64_dpp %0, -%1, %2 row_newbcast:15 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %3 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %4 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %5 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %7 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %8 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %9 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %10 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %11 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %12 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %13 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %14 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %15 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %16 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %0, -%1, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf"
+      "v_fmac_f64_dpp %0, -%16, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, -%16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, -%16, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, -%16, %17 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, -%16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, -%16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %7, -%16, %17 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %8, -%16, %17 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %9, -%16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %10, -%16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %11, -%16, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %12, -%16, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %13, -%16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %14, -%16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %15, -%16, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf"
+      : "+v"(p[(J - 1) & 15]), "+v"(p[(J - 2) & 15]), "+v"(p[(J - 3) & 15]), "+v"(p[(J - 4) & 15]),
+        "+v"(p[(J - 5) & 15]), "+v"(p[(J - 6) & 15]), "+v"(p[(J - 7) & 15]), "+v"(p[(J - 8) & 15]),
+        "+v"(p[(J - 9) & 15]), "+v"(p[(J - 10) & 15]), "+v"(p[(J - 11) & 15]), "+v"(p[(J - 12) & 15]),
+        "+v"(p[(J - 13) & 15]), "+v"(p[(J - 14) & 15]), "+v"(p[(J - 15) & 15]), "+v"(fresh)
+      : "v"(a), "v"(lam));
+  p[J] = fresh;
+  return lam;
+}
+
+// acc += v(lane K of the row) * w: a row broadcast of a per-lane value. The
+// s_nop 1 covers a v produced by the VALU right before (a DPP source needs two
+// wait states).
+template <int K>
+__device__ __forceinline__ void fma_bcast(double& acc, double v, double w) {
+  asm("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
       : "+v"(acc)
-      : "v"(a), "v"(r[(J - 16) & 15]), "v"(r[(J - 15) & 15]), "v"(r[(J - 14) & 15]),
-        "v"(r[(J - 13) & 15]), "v"(r[(J - 12) & 15]), "v"(r[(J - 11) & 15]),
-        "v"(r[(J - 10) & 15]), "v"(r[(J - 9) & 15]), "v"(r[(J - 8) & 15]), "v"(r[(J - 7) & 15]),
-        "v"(r[(J - 6) & 15]), "v"(r[(J - 5) & 15]), "v"(r[(J - 4) & 15]), "v"(r[(J - 3) & 15]),
-        "v"(r[(J - 2) & 15]), "v"(r[(J - 1) & 15]));
-  return acc;
+      : "v"(v), "v"(w), "i"(K));
 }
 
 // Carry link on lanes 0-15 (lane r owns s_r): returns z_r + sum_k M[r][k] s_k in
