@@ -27,15 +27,16 @@
 
 namespace tlpc {
 
-template <typename S, int NP>
+// LANES: lanes per summary item (the forward summary uses NP, the adjoint NP+1).
+template <typename S, int NP, int LANES = NP + 1>
 struct ChunkCfg {
   static constexpr int TB = NP < 16 ? 16 : NP;  // time steps staged per LDS block
   static constexpr int LDS_BUDGET = 40 * 1024;
-  static constexpr int G_LANES = 256 / (NP + 1);
+  static constexpr int G_LANES = 256 / LANES;
   using W = typename Wide<S>::type;  // summaries and carries
   static constexpr int G_LDS = LDS_BUDGET / (TB * (NP + 1) * (int)sizeof(W));
   static constexpr int G = (G_LANES < G_LDS ? G_LANES : G_LDS) < 1 ? 1 : (G_LANES < G_LDS ? G_LANES : G_LDS);
-  static constexpr int THREADS = ((G * (NP + 1) + 63) / 64) * 64;
+  static constexpr int THREADS = ((G * LANES + 63) / 64) * 64;
   static constexpr int ASTRIDE = TB * NP + (16 / (int)sizeof(W) > 0 ? 16 / (int)sizeof(W) : 1);
   static constexpr int SUMM = NP * NP + NP;  // elements per chunk summary
 };
@@ -46,18 +47,19 @@ struct ChunkCfg {
 // and the chunks without a summary (the last of a signal forward, the first
 // adjoint) are idle. Per item the workgroup keeps, in LDS, where the chunk's A
 // rows and inputs start and how many of its samples exist.
-template <typename S, int NP>
+template <typename S, int NP, int LANES = NP + 1>
 struct ItemTab {
-  int64_t a0[ChunkCfg<S, NP>::G];   // offset of A at the chunk's first sample
-  int64_t at[ChunkCfg<S, NP>::G];   // A stride over time
-  int64_t x0[ChunkCfg<S, NP>::G];   // offset of the chunk's first input sample
-  int64_t rem[ChunkCfg<S, NP>::G];  // samples of the signal from the chunk start (<= 0: idle)
+  static constexpr int G = ChunkCfg<S, NP, LANES>::G;
+  int64_t a0[G];   // offset of A at the chunk's first sample
+  int64_t at[G];   // A stride over time
+  int64_t x0[G];   // offset of the chunk's first input sample
+  int64_t rem[G];  // samples of the signal from the chunk start (<= 0: idle)
 };
 
-template <typename S, int NP>
-__device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP>& tab, int64_t first,
+template <typename S, int NP, int LANES>
+__device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP, LANES>& tab, int64_t first,
                                            bool fwd) {
-  using C = ChunkCfg<S, NP>;
+  using C = ChunkCfg<S, NP, LANES>;
   const int gg = threadIdx.x;
   if (gg >= C::G) return;
   const int64_t item = first + gg;
@@ -91,11 +93,11 @@ __device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP>
 // Staging helper: copy the A rows and the per-step input of the G items for the
 // time block [t0, t0 + TB) into LDS; rows beyond the signal end and lags >= n
 // are zero.
-template <typename S, int NP>
+template <typename S, int NP, int LANES>
 __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide<S>::type* sA,
                                             typename Wide<S>::type* sX, const S* inp,
-                                            const ItemTab<S, NP>& tab, int t0) {
-  using C = ChunkCfg<S, NP>;
+                                            const ItemTab<S, NP, LANES>& tab, int t0) {
+  using C = ChunkCfg<S, NP, LANES>;
   using W = typename Wide<S>::type;
   const int nthr = blockDim.x;
   for (int idx = threadIdx.x; idx < C::G * C::TB * NP; idx += nthr) {
@@ -117,56 +119,59 @@ __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide
 }
 
 // ----------------------------------------------------------------------------
-// Phase 1 (forward): summaries of chunks c = 0 .. P-2 of every signal.
+// Phase 1 (forward): summaries of chunks c = 0 .. P-2 of every signal, NP lanes
+// per chunk. Lane k sweeps the chunk backwards with the transposed recurrence
+// from a unit at output k (the sample L-1-k of the chunk): the pending sums left
+// for the positions before the chunk are d y_{L-1-k} / d y_{-1-j} = Phi[k][j]
+// (complex-linear: no conjugate), and the lambdas met on the way are
+// d y_{L-1-k} / d x_t, so z[k] = sum_t lambda_t x_t. Each step's NP updates are
+// independent of each other (the forward form chains NP products per step), and
+// no extra lane carries the zero-state response.
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
-  using C = ChunkCfg<S, NP>;
+  using C = ChunkCfg<S, NP, NP>;
   using W = typename Wide<S>::type;
   __shared__ W sA[C::G * C::ASTRIDE];
   __shared__ W sX[C::G * C::TB];
-  __shared__ ItemTab<S, NP> tab;
+  __shared__ ItemTab<S, NP, NP> tab;
   const int64_t first = (int64_t)blockIdx.x * C::G;
-  fill_items<S, NP>(a, tab, first, true);
+  fill_items<S, NP, NP>(a, tab, first, true);
   __syncthreads();
-  const int g = threadIdx.x / (NP + 1);
-  const int k = threadIdx.x - g * (NP + 1);  // k < NP: basis e_k; k == NP: zero state
+  const int g = threadIdx.x / NP;
+  const int k = threadIdx.x - g * NP;
   const bool active = g < C::G && tab.rem[g] > 0;
 
-  W ring[NP];
+  W ring[NP];  // pending sums, slot u mod NP for position u
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[r] = Num<W>::zero();
-  // s_in = e_k means y_{-1-k} = 1, which lives in slot (-1-k) mod NP.
-#pragma unroll
-  for (int r = 0; r < NP; ++r)
-    if (k == r) ring[NP - 1 - r] = Num<W>::one();
-  const W xmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
+  for (int r = 0; r < NP; ++r) ring[r] = (r == NP - 1 - k) ? Num<W>::one() : Num<W>::zero();
+  W z = Num<W>::zero();
 
-  for (int t0 = 0; t0 < a.L; t0 += C::TB) {
-    stage_block<S, NP>(a, sA, sX, a.x, tab, t0);
+  for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
+    stage_block<S, NP, NP>(a, sA, sX, a.x, tab, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
 #pragma unroll
-      for (int j = 0; j < C::TB; ++j) {
+      for (int jj = 0; jj < C::TB; ++jj) {
+        const int j = C::TB - 1 - jj;  // local time; slot of position t is j mod NP
         const W* row = rowbase + j * NP;
-        W acc = sX[g * C::TB + j] * xmask;
+        const int h = j & (NP - 1);
+        const W lam = ring[h];
+        ring[h] = Num<W>::zero();
 #pragma unroll
-        for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
-        ring[j & (NP - 1)] = acc;
+        for (int i = 1; i <= NP; ++i)
+          ring[(j - i) & (NP - 1)] = Num<W>::fms(ring[(j - i) & (NP - 1)], row[i - 1], lam);
+        z = Num<W>::fma(z, lam, sX[g * C::TB + j]);
       }
     }
     __syncthreads();
   }
   if (active) {
     W* out = a.summ + (first + g) * C::SUMM;
-    // final state s[r] = y_{L-1-r}, slot (L-1-r) mod NP = NP-1-r
-    if (k < NP) {
+    // pending for position -1-r (before the chunk) sits in slot NP-1-r
 #pragma unroll
-      for (int r = 0; r < NP; ++r) out[r * NP + k] = ring[NP - 1 - r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < NP; ++r) out[NP * NP + r] = ring[NP - 1 - r];
-    }
+    for (int r = 0; r < NP; ++r) out[k * NP + r] = ring[NP - 1 - r];
+    out[NP * NP + k] = z;
   }
 }
 
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   __shared__ W sX[C::G * C::TB];
   __shared__ ItemTab<S, NP> tab;
   const int64_t first = (int64_t)blockIdx.x * C::G;
-  fill_items<S, NP>(a, tab, first, false);
+  fill_items<S, NP, NP + 1>(a, tab, first, false);
   __syncthreads();
   const int g = threadIdx.x / (NP + 1);
   const int k = threadIdx.x - g * (NP + 1);
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   const W gmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
 
   for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
-    stage_block<S, NP>(a, sA, sX, a.x, tab, t0);
+    stage_block<S, NP, NP + 1>(a, sA, sX, a.x, tab, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
@@ -1051,8 +1056,9 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   if (a.P > 1) {
-    const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
-    hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
+    using CF = ChunkCfg<S, NP, NP>;
+    const int64_t grid = (summary_items(a) + CF::G - 1) / CF::G;
+    hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(CF::THREADS), 0, st, a);
   }
   bool blocked = false;
   if constexpr (blocked_fits<S, NP>()) {
