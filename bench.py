@@ -536,6 +536,24 @@ def main():
         },
     }
 
+    # a device copy of COPY_BYTES, timed the same way: the bandwidth a plain
+    # streaming kernel reaches on this GPU, reported beside the published peak
+    src = torch.ones(COPY_BYTES // 4, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(10):
+        dst.copy_(src)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * COPY_BYTES * 10 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del src, dst
+    result["roofline"]["measured_copy_gbs"] = copy_gbs
+    result["roofline"]["frac_of_copy"] = achieved / copy_gbs
+    log(f"[bench] device copy {copy_gbs:.0f} GB/s (read + write)")
+
     if args.calibrate_copy:
         src = torch.ones(COPY_BYTES // 4, device=dev)
         dst = torch.empty_like(src)

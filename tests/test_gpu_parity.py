@@ -258,9 +258,10 @@ def test_backward_reuses_forward_summaries(T, chunk):
     assert all(torch.equal(a, b) for a, b in zip(again, first))
 
 
-def test_order16_large_batch_two_streams():
-    """Order 16, float32, B >= 64: the batch runs as two halves on the caller's
-    stream and a side stream; every output meets the FP64 oracle."""
+def test_order16_large_batch():
+    """Order 16, float32, a batch of 67 signals (more than one summary workgroup
+    of 16 chunks per signal group, a ragged last workgroup): every output meets
+    the FP64 oracle, with and without the forward's kept summaries."""
     rng = np.random.default_rng(66)
     B, T = 67, 2500
     x, A, zi = stable_inputs(rng, B, T, 16, np.float32, kmax=0.8, hop=240)
@@ -293,6 +294,49 @@ def test_autograd_order16_uses_kept_summaries():
     assert rel(host(X.grad), rx) <= 1e-5
     assert rel(host(AA.grad), rA) <= 1e-5
     assert rel(host(Z.grad), rzi) <= 1e-5
+
+
+def _offset_view(a):
+    """A contiguous device copy of `a` that starts 4 bytes into its allocation, so
+    its rows are not 16-byte aligned."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    buf = torch.empty(t.numel() + 1, dtype=t.dtype, device="cuda")
+    v = buf[1:].view(t.shape)
+    v.copy_(t.cuda())
+    assert v.is_contiguous() and v.data_ptr() % 16 != 0
+    return v
+
+
+@pytest.mark.parametrize("T", [1000, 1001, 4096])
+def test_order16_unaligned_views(T):
+    """Order-16 float32 tensors that start mid-allocation: A rows that are not
+    16-byte aligned take the generic kernels (the order-16 kernels read rows as
+    float4), and x, g, y views that are not aligned take per-sample loads. Forward,
+    backward and autograd (kept summaries) all meet the FP64 oracle."""
+    import torchlpc_amd
+
+    rng = np.random.default_rng(T)
+    x, A, zi = stable_inputs(rng, 3, T, 16, np.float32, kmax=0.8, hop=240)
+    g = stable_inputs(rng, 3, T, 1, np.float32)[0]
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    ref = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    for xa, Aa, ga in ((True, False, True), (False, True, False), (False, False, False)):
+        X = dev(x) if xa else _offset_view(x)
+        AA = dev(A) if Aa else _offset_view(A)
+        G = dev(g) if ga else _offset_view(g)
+        y = ops().lpc(X, AA, dev(zi))
+        assert rel(host(y), y64) <= 1e-5, (xa, Aa)
+        outs = ops().lpc_backward(G, AA, dev(zi), y, True, True, True)
+        for o, r in zip(outs, ref):
+            assert rel(host(o), r) <= 1e-5, (xa, Aa, ga)
+        # leaves that share the (possibly unaligned) storage of the views
+        Xg, Ag, Zg = X.detach().requires_grad_(), AA.detach().requires_grad_(), dev(zi).requires_grad_()
+        assert Ag.data_ptr() == AA.data_ptr()
+        yy = torchlpc_amd.LPC.apply(Xg, Ag, Zg)
+        yy.backward(G)
+        assert rel(host(yy), y64) <= 1e-5
+        for o, r in zip((Xg.grad, Ag.grad, Zg.grad), ref):
+            assert rel(host(o), r) <= 1e-5, (xa, Aa, ga)
 
 
 @pytest.mark.parametrize("T,chunk", [(1, 0), (15, 0), (17, 16), (1000, 16), (3583, 64), (3585, 64), (48000, 0), (20000, 32)])

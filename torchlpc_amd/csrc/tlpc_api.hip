@@ -277,8 +277,13 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
     a.dA = static_cast<S*>(dA);
   }
   if constexpr (sizeof(S) == 4) {
-    // float32, order 16, time-varying: DPP summaries and quad reruns
-    if (p.c16) {
+    // float32, order 16, time-varying: DPP summaries and quad reruns. They read
+    // A (and write dA) as one float4 per lane, so both must be 16-byte aligned;
+    // a view that starts mid-row takes the generic kernels (same plan and
+    // workspace, which hold summaries of the same size).
+    const bool rows_aligned = (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+                              (reinterpret_cast<uintptr_t>(a.dA) & 15) == 0;
+    if (p.c16 && rows_aligned) {
       int rc = TLPC_OK;
       if (!backward) {
         rc = launch_chunk16_fwd_summary(a, true, st);

@@ -580,11 +580,11 @@ __global__ __launch_bounds__(RERUN_THREADS) void rerun_adj(ChunkArgs<float> a) {
 // chunk P-1-l, M = Phi_{P-1-l}^T, v = w_{P-1-l}; the first link (the last chunk)
 // meets the zero state past the end of the signal, so it needs no matrix.
 //
-// Wave 0 chains; waves 1..3 load. Links come in batches of CB; batch j is loaded
-// by wave 1 + j % 3, which issues its global loads DEPTH = 3 batches before the
-// chain needs it and writes it to one of two LDS buffers in the iteration just
-// before. One barrier per batch keeps the four waves in step, so the chain never
-// waits for memory once the pipeline is full.
+// Wave 0 chains; the other LOADERS waves load. Links come in batches of CB;
+// batch j is loaded by wave 1 + j % LOADERS, which issues its global loads
+// DEPTH = LOADERS batches before the chain needs it and writes it to one of two
+// LDS buffers in the iteration just before. One barrier per batch keeps the
+// waves in step, so the chain never waits for memory once the pipeline is full.
 constexpr int CB = 8;
 constexpr int SUMM = NP * NP + NP;
 // LDS layout of one link: 16 matrix rows padded to 17 doubles (row r at r*17,
@@ -593,7 +593,8 @@ constexpr int SUMM = NP * NP + NP;
 constexpr int ROWP = NP + 1;
 constexpr int LINKP = NP * ROWP + NP;
 constexpr int CARRY_THREADS = 256;
-constexpr int DEPTH = 3;
+constexpr int LOADERS = CARRY_THREADS / 64 - 1;
+constexpr int DEPTH = LOADERS;
 
 template <bool FWD>
 __global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
@@ -649,9 +650,9 @@ __global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
   for (int64_t i = -DEPTH; i < nbatch; ++i) {
     if (wave > 0) {
       const int64_t jn = i + DEPTH;  // issue
-      if (jn < nbatch && 1 + jn % 3 == wave) fetch(jn);
+      if (jn < nbatch && 1 + jn % LOADERS == wave) fetch(jn);
       const int64_t jc = i + 1;  // commit (loaded DEPTH - 1 iterations ago)
-      if (jc >= 0 && jc < nbatch && 1 + jc % 3 == wave) commit(jc);
+      if (jc >= 0 && jc < nbatch && 1 + jc % LOADERS == wave) commit(jc);
     } else if (i >= 0) {
       const int64_t l0 = i * CB;
       const int nb = (int)((links - l0) < CB ? (links - l0) : CB);
