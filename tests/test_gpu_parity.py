@@ -108,9 +108,13 @@ def test_backward_serial_strategy(dtype):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("N", [2, 16, 32])
-def test_time_invariant(dtype, N):
-    rng = np.random.default_rng(N)
-    B, T = 4, 3000
+@pytest.mark.parametrize("T", [3000, 2998, 17])
+def test_time_invariant(dtype, N, T):
+    """a[B, N] and its time-broadcast view. T = 3000 takes the float4 streams of
+    the float32 kernels, T = 2998 (not a multiple of 4) the per-sample ones, and
+    T = 17 is a single chunk."""
+    rng = np.random.default_rng(N + T)
+    B = 4
     # k ~ U(-0.8, 0.8) puts poles within 1e-8 of the unit circle at N = 32, where
     # the serial reference loop itself loses 1e-4 (float32) / 1e-11 (float64) to
     # rounding; min(0.8, 6/N) keeps the comparison well conditioned.
@@ -305,6 +309,21 @@ def _offset_view(a):
     v.copy_(t.cuda())
     assert v.is_contiguous() and v.data_ptr() % 16 != 0
     return v
+
+
+def test_time_invariant_unaligned_views():
+    """Time-invariant float32 streams that start 4 bytes into their allocation
+    take the per-sample loads; results match the aligned call bit for bit."""
+    rng = np.random.default_rng(5)
+    B, T, N = 3, 2000, 32
+    a = O.step_up(rng.uniform(-0.15, 0.15, (B, N))).astype(np.float32)
+    x, zi, g = (rng.standard_normal(s).astype(np.float32) for s in ((B, T), (B, N), (B, T)))
+    y = ops().lpc(dev(x), dev(a), dev(zi))
+    yu = ops().lpc(_offset_view(x), dev(a), dev(zi))
+    assert torch.equal(y, yu)
+    ref = ops().lpc_backward(dev(g), dev(a), dev(zi), y, True, True, True)
+    got = ops().lpc_backward(_offset_view(g), dev(a), dev(zi), yu, True, True, True)
+    assert all(torch.equal(r, o) for r, o in zip(ref, got))
 
 
 @pytest.mark.parametrize("T", [1000, 1001, 4096])

@@ -431,15 +431,27 @@ __device__ __forceinline__ void adj_quad_run(double (&P)[4], double& now, const 
 // Lane kk holds A[t, kk+1] of the step (a 16-slot ring, consumed then reloaded
 // for the block below: the row base np is clamped once per block so the last
 // prefetch stays in the signal) and x of step tb+kk of the block.
+// Steps 4M+3 .. 4M of a 16-step block from ring slot M: the float4 of rows
+// tb+4M .. tb+4M+3 (lane kk holds row tb+4M+(kk>>2), coefficients 4(kk&3) ..
+// 4(kk&3)+3). The slot is converted, then reloaded with the block below.
 template <int J, bool WITH_Z>
-__device__ __forceinline__ void adj_row_block(double (&p)[NP], double& z, float (&cf)[NP], double xv,
-                                              const float* np) {
+__device__ __forceinline__ void adj_row_quad(double (&p)[NP], double& z, const double (&d)[4], double xv) {
   if constexpr (J >= 0) {
-    const double lam = adj_row_step<J>(p, (double)cf[J]);
+    const double lam = adj_row_step4<J>(p, d);
     if constexpr (WITH_Z) fma_bcast<J>(z, xv, lam);
-    cf[J] = np[J * NP];
+    if constexpr ((J & 3) != 0) adj_row_quad<J - 1, WITH_Z>(p, z, d, xv);
+  }
+}
+template <int M, bool WITH_Z>
+__device__ __forceinline__ void adj_row_block(double (&p)[NP], double& z, float4 (&cq)[4], double xv,
+                                              const float4* nq) {
+  if constexpr (M >= 0) {
+    const double d[4] = {(double)cq[M].x, (double)cq[M].y, (double)cq[M].z, (double)cq[M].w};
+    cq[M] = nq[M * 16];
     pin_loads();
-    adj_row_block<J - 1, WITH_Z>(p, z, cf, xv, np);
+    adj_row_quad<4 * M + 3, WITH_Z>(p, z, d, xv);
+    pin_loads();
+    adj_row_block<M - 1, WITH_Z>(p, z, cq, xv, nq);
   }
 }
 
@@ -456,22 +468,23 @@ __global__ __launch_bounds__(256) void sum_fwd(ChunkArgs<float> a) {
   const int64_t b = cid / cps;
   const int c = (int)(cid - b * cps);
   const int t0 = c * L;
-  const float* ap = a.A + (b * a.T + t0) * NP + kk;
+  // lane kk reads float4 kk of a 4-row group: row (kk >> 2), coefficients 4(kk & 3)..
+  const float4* ap = reinterpret_cast<const float4*>(a.A + (b * a.T + t0) * NP) + kk;
   const float* xp = a.x + b * a.T + t0 + kk;
   double p[NP];
 #pragma unroll
   for (int i = 0; i < NP; ++i) p[i] = (i == NP - 1 - kk) ? 1.0 : 0.0;  // unit at t0+L-1-kk
   double z = 0.0;
-  float cf[NP];
+  float4 cq[4];
 #pragma unroll
-  for (int j = 0; j < NP; ++j) cf[j] = opaque(ap[(L - NP + j) * NP]);
+  for (int m = 0; m < 4; ++m) cq[m] = opaque(ap[(L - NP + 4 * m) * 4]);
   float xf = WITH_Z ? opaque(xp[L - NP]) : 0.f;
   for (int jb = L - NP; jb >= 0; jb -= NP) {
     const int jn = jb >= NP ? jb - NP : 0;  // the block below (block 0 again after the last)
     const double xv = (double)xf;
     float xn = 0.f;
     if constexpr (WITH_Z) xn = xp[jn];
-    adj_row_block<NP - 1, WITH_Z>(p, z, cf, xv, ap + jn * NP);
+    adj_row_block<3, WITH_Z>(p, z, cq, xv, ap + jn * 4);
     xf = xn;
   }
   double* out = a.summ + cid * (NP * NP + NP);

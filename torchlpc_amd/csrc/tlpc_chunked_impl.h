@@ -22,6 +22,8 @@
 // at a multiple of NP, and the unrolled inner loop covers NP steps.
 #pragma once
 
+#include <type_traits>
+
 #include "tlpc_common.h"
 #include "tlpc_kernels.h"
 
@@ -520,6 +522,34 @@ __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Workgroup barrier that orders LDS only: __syncthreads() also fences global
+// memory, which waits for every load in flight (a prefetch ring).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// float4 streams of the time-invariant float32 reruns: component K of a block of
+// four consecutive samples, loaded and stored whole.
+template <int K>
+__device__ __forceinline__ float v4get(const float4& v) {
+  if constexpr (K == 0) return v.x;
+  if constexpr (K == 1) return v.y;
+  if constexpr (K == 2) return v.z;
+  return v.w;
+}
+template <int K>
+__device__ __forceinline__ void v4set(float4& v, float f) {
+  if constexpr (K == 0) v.x = f;
+  if constexpr (K == 1) v.y = f;
+  if constexpr (K == 2) v.z = f;
+  if constexpr (K == 3) v.w = f;
+}
+__device__ __forceinline__ float4 ld4(const float* p, int64_t i) {
+  return *reinterpret_cast<const float4*>(p + i);
+}
+
 // Keeps a step's loads where the source puts them (see tlpc_chunk16.hip).
 __device__ __forceinline__ void pin_step() { __builtin_amdgcn_sched_barrier(0); }
 
@@ -547,37 +577,59 @@ __device__ __forceinline__ T opaque_v(T v) {
   return v;
 }
 
-template <typename S, int NP>
+// VEC (float32 data, T a multiple of 4, x and y 16-byte aligned): x is read and
+// y written as one float4 per four steps instead of one 4-byte access per step;
+// each 4-byte access of a lane is a request of its own, and those requests, not
+// the bytes, bound the scalar form.
+template <typename S, int NP, bool VEC>
 struct FwdLaneTI {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
-  W ring[NP];    // y_u in slot u mod NP
-  W cw[NP];      // coefficients
-  S xs[RC::DS];  // x_t in slot t mod DS
+  W ring[NP];                        // y_u in slot u mod NP
+  W cw[NP];                          // coefficients
+  S xs[VEC ? 1 : RC::DS];            // x_t in slot t mod DS
+  float4 xq[VEC ? RC::DS / 4 : 1];   // VEC: x[4q' .. 4q'+3] in slot q' mod DS/4
+  float4 yq;                         // VEC: the outputs of the current four steps
   const S* xb;
   S* yb;
   int64_t T, tend;
 };
 
-template <typename S, int NP, int JJ>
-__device__ __forceinline__ void fwd_steps_ti(FwdLaneTI<S, NP>& st, int64_t tb) {
+template <typename S, int NP, bool VEC, int JJ>
+__device__ __forceinline__ void fwd_steps_ti(FwdLaneTI<S, NP, VEC>& st, int64_t tb) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
   if constexpr (JJ < RC::K) {
     constexpr int j = JJ % NP;  // t mod NP: blocks start at multiples of NP
     const int64_t t = tb + JJ;
-    W acc = convert<W>(st.xs[JJ % RC::DS]);
+    W acc;
+    if constexpr (VEC)
+      acc = convert<W>(v4get<JJ & 3>(st.xq[(JJ % RC::DS) / 4]));
+    else
+      acc = convert<W>(st.xs[JJ % RC::DS]);
 #pragma unroll
     for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, st.cw[i - 1], st.ring[(j - i) & (NP - 1)]);
-    st.xs[JJ % RC::DS] = st.xb[t + RC::DS < st.T ? t + RC::DS : st.T - 1];
+    if constexpr (VEC) {
+      // the block's last sample was just read: its slot takes the block DS ahead
+      if constexpr ((JJ & 3) == 3) st.xq[(JJ % RC::DS) / 4] = ld4(st.xb, min(t - 3 + RC::DS, st.T - 4));
+    } else {
+      st.xs[JJ % RC::DS] = st.xb[t + RC::DS < st.T ? t + RC::DS : st.T - 1];
+    }
     pin_step();
     st.ring[j] = acc;
-    if (t < st.tend) st.yb[t] = convert<S>(acc);
-    fwd_steps_ti<S, NP, JJ + 1>(st, tb);
+    if constexpr (VEC) {
+      v4set<JJ & 3>(st.yq, (float)acc);
+      // chunk ends are multiples of 4, so a block is wholly inside or past the end
+      if constexpr ((JJ & 3) == 3)
+        if (t < st.tend) *reinterpret_cast<float4*>(st.yb + t - 3) = st.yq;
+    } else {
+      if (t < st.tend) st.yb[t] = convert<S>(acc);
+    }
+    fwd_steps_ti<S, NP, VEC, JJ + 1>(st, tb);
   }
 }
 
-template <typename S, int NP>
+template <typename S, int NP, bool VEC>
 __global__ __launch_bounds__(256) void fwd_rerun_ti(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
@@ -588,18 +640,23 @@ __global__ __launch_bounds__(256) void fwd_rerun_ti(ChunkArgs<S> a) {
   const int64_t t0 = c * a.L;
   const int64_t rem = a.T - t0;
   const int len = rem < a.L ? (int)rem : a.L;
-  FwdLaneTI<S, NP> st;
+  FwdLaneTI<S, NP, VEC> st;
   st.xb = a.x + b * a.T;
   st.yb = a.y + b * a.T;
   st.T = a.T;
   st.tend = t0 + len;
+  if constexpr (VEC) {
 #pragma unroll
-  for (int k = 0; k < RC::DS; ++k) st.xs[k] = opaque_v(st.xb[t0 + k < a.T ? t0 + k : a.T - 1]);
+    for (int q = 0; q < RC::DS / 4; ++q) st.xq[q] = opaque_v(ld4(st.xb, min(t0 + 4 * q, a.T - 4)));
+  } else {
+#pragma unroll
+    for (int k = 0; k < RC::DS; ++k) st.xs[k] = opaque_v(st.xb[t0 + k < a.T ? t0 + k : a.T - 1]);
+  }
   load_coefs_w<S, NP>(st.cw, a, a.A + b * a.a_bstride);
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = opaque_v(cr[r]);
-  for (int jb = 0; jb < len; jb += RC::K) fwd_steps_ti<S, NP, 0>(st, t0 + jb);
+  for (int jb = 0; jb < len; jb += RC::K) fwd_steps_ti<S, NP, VEC, 0>(st, t0 + jb);
 }
 
 // y_u of one signal for the adjoint's window: zi for u < 0 (k < n), zero past T,
@@ -623,7 +680,9 @@ __device__ __forceinline__ bool yw_ok(const S* zib, int n, int64_t T, int64_t u)
 // accumulates the lane's share of da.
 enum DaMode { DA_NONE = 0, DA_PARTIAL = 2 };
 
-template <typename S, int NP>
+// VEC (float32 data, T a multiple of 4, g and dx 16-byte aligned): g is read and
+// dx written as one float4 per four steps.
+template <typename S, int NP, bool VEC>
 struct BwdLaneTI {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
@@ -631,7 +690,9 @@ struct BwdLaneTI {
   W cw[NP];      // conj of the coefficients
   S yr[NP];      // conj(y) window: slot u mod NP holds y_u for u in [t-NP, t-1] (exact)
   W dacc[NP];    // da partials
-  S gs[RC::DS];  // g_t in slot (tstart - t) mod DS
+  S gs[VEC ? 1 : RC::DS];           // g_t in slot (tstart - t) mod DS
+  float4 gq[VEC ? RC::DS / 4 : 1];  // VEC: g[u .. u+3] (u a multiple of 4), by step block
+  float4 dq;                        // VEC: lambda of the current four steps
   S ys[RC::DS];  // y_{t-1-NP} (window input of step t), same slots
   const S* gb;
   const S* yb;
@@ -645,23 +706,36 @@ struct BwdLaneTI {
 // runtime index would put the float64 ring in scratch memory). DA is a template
 // argument: a runtime choice inside the steps splits them into branches, and the
 // wait-count pass then drains the loads in flight.
-template <typename S, int NP, int DA, int JJ>
-__device__ __forceinline__ void bwd_steps_ti(const ChunkArgs<S>& a, BwdLaneTI<S, NP>& st, int64_t tb) {
+template <typename S, int NP, int DA, bool VEC, int JJ>
+__device__ __forceinline__ void bwd_steps_ti(const ChunkArgs<S>& a, BwdLaneTI<S, NP, VEC>& st, int64_t tb) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
   if constexpr (JJ < RC::K) {
     constexpr int j = (NP - 1 - JJ % NP);
-    const int64_t t = tb - JJ;
+    const int64_t t = tb - JJ;  // tb is 3 mod 4: step JJ reads component 3 - JJ % 4
     const bool live = t < st.T;
-    const W lam0 = convert<W>(st.gs[JJ % RC::DS]) + st.ring[j];
+    W gval;
+    if constexpr (VEC)
+      gval = convert<W>(v4get<3 - (JJ & 3)>(st.gq[(JJ % RC::DS) / 4]));
+    else
+      gval = convert<W>(st.gs[JJ % RC::DS]);
+    const W lam0 = gval + st.ring[j];
     const W lam = live ? lam0 : Num<W>::zero();
     st.ring[j] = live ? Num<W>::zero() : st.ring[j];
 #pragma unroll
     for (int i = 1; i <= NP; ++i)
       st.ring[(j - i) & (NP - 1)] = Num<W>::fms(st.ring[(j - i) & (NP - 1)], st.cw[i - 1], lam);
-    // the slot just consumed takes the step DS further down
-    st.gs[JJ % RC::DS] = st.gb[clamp64(t - RC::DS, 0, st.T - 1)];
-    if (live && st.dxb) st.dxb[t] = convert<S>(lam);
+    // the slot just consumed takes the step (VEC: the block) DS further down
+    if constexpr (VEC) {
+      if constexpr ((JJ & 3) == 3) st.gq[(JJ % RC::DS) / 4] = ld4(st.gb, clamp64(t - RC::DS, 0, st.T - 4));
+      v4set<3 - (JJ & 3)>(st.dq, (float)lam);
+      // t is a multiple of 4 here and T is too: the block is all live or all past T
+      if constexpr ((JJ & 3) == 3)
+        if (live && st.dxb) *reinterpret_cast<float4*>(st.dxb + t) = st.dq;
+    } else {
+      st.gs[JJ % RC::DS] = st.gb[clamp64(t - RC::DS, 0, st.T - 1)];
+      if (live && st.dxb) st.dxb[t] = convert<S>(lam);
+    }
     if constexpr (DA == DA_PARTIAL) {
 #pragma unroll
       for (int i = 1; i <= NP; ++i)
@@ -673,14 +747,14 @@ __device__ __forceinline__ void bwd_steps_ti(const ChunkArgs<S>& a, BwdLaneTI<S,
       st.ys[JJ % RC::DS] = *yw_addr(st.yb, st.zib, a.n, st.T, t - RC::DS - 1 - NP);
     }
     pin_step();
-    bwd_steps_ti<S, NP, DA, JJ + 1>(a, st, tb);
+    bwd_steps_ti<S, NP, DA, VEC, JJ + 1>(a, st, tb);
   }
 }
 
 // Adjoint rerun, time-invariant: one lane per chunk walks its samples backwards
 // from the carry, writes dx and (DA_PARTIAL) the chunk's partial of da; chunk 0
 // writes dzi.
-template <typename S, int NP, int DA>
+template <typename S, int NP, int DA, bool VEC>
 __global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
@@ -689,7 +763,7 @@ __global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
   const int64_t b = cid / a.P;
   const int64_t c = cid - b * a.P;
   const int64_t t0 = c * a.L;
-  BwdLaneTI<S, NP> st;
+  BwdLaneTI<S, NP, VEC> st;
   st.gb = a.x + b * a.T;  // upstream gradient
   st.yb = a.y + b * a.T;
   st.zib = a.zi ? a.zi + b * a.n : nullptr;
@@ -699,8 +773,12 @@ __global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
 #pragma unroll
   for (int k = 0; k < RC::DS; ++k) {
     const int64_t t = tstart - k;
-    st.gs[k] = opaque_v(st.gb[clamp64(t, 0, a.T - 1)]);
+    if constexpr (!VEC) st.gs[k] = opaque_v(st.gb[clamp64(t, 0, a.T - 1)]);
     if constexpr (DA == DA_PARTIAL) st.ys[k] = opaque_v(*yw_addr(st.yb, st.zib, a.n, a.T, t - 1 - NP));
+  }
+  if constexpr (VEC) {
+#pragma unroll
+    for (int q = 0; q < RC::DS / 4; ++q) st.gq[q] = opaque_v(ld4(st.gb, clamp64(tstart - 3 - 4 * q, 0, a.T - 4)));
   }
   load_coefs_w<S, NP>(st.cw, a, a.A + b * a.a_bstride);
 #pragma unroll
@@ -718,7 +796,7 @@ __global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
   const W* cr = a.carry + cid * NP;
 #pragma unroll
   for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = opaque_v(cr[r]);
-  for (int jb = 0; jb < a.L; jb += RC::K) bwd_steps_ti<S, NP, DA, 0>(a, st, tstart - jb);
+  for (int jb = 0; jb < a.L; jb += RC::K) bwd_steps_ti<S, NP, DA, VEC, 0>(a, st, tstart - jb);
   if constexpr (DA == DA_PARTIAL) {
     W* dp = a.dapart + cid * NP;
 #pragma unroll
@@ -900,6 +978,8 @@ __global__ __launch_bounds__(256) void ti_da_reduce(ChunkArgs<S> a) {
   const int i = (int)(idx - b * a.n);
   W acc = Num<W>::zero();
   const W* dp = a.dapart + b * a.P * NP + i;
+  // unrolled so that 16 loads are in flight; the sum keeps chunk order
+#pragma unroll 16
   for (int64_t c = 0; c < a.P; ++c) acc = acc + dp[c * NP];
   a.da_ti[b * a.n + i] = convert<S>(acc);
 }
@@ -925,52 +1005,102 @@ __device__ __forceinline__ void load_coefs(typename Wide<S>::type (&co)[NP], con
   }
 }
 
-template <typename S, int NP, bool FWD>
+// Zero-state response of one chunk, both directions in the pending-sum form: the
+// step's NP products are independent, and only the first feeds the next step.
+// Forward step t: y = x_t + p(t), then p(t+i) -= a_i y (i = 1..NP) and y joins
+// the history window yh. Adjoint step t (descending): lambda = g_t + p(t), then
+// p(t-i) -= conj(a_i) lambda. The input stream is prefetched DS steps ahead in a
+// ring whose slots are consumed before they are reloaded; with VEC (float32,
+// T a multiple of 4, input 16-byte aligned) as float4 blocks.
+template <typename S, int NP, bool VEC>
+struct ZsumLane {
+  using W = typename Wide<S>::type;
+  using RC = RerunCfg<S, NP>;
+  W p[NP];                          // pending sums, slot u mod NP for position u
+  W yh[NP];                         // forward: y_u in slot u mod NP
+  W co[NP];                         // coefficients (conjugated for the adjoint)
+  S xs[VEC ? 1 : RC::DS];           // input sample in slot JJ mod DS
+  float4 xq[VEC ? RC::DS / 4 : 1];  // VEC: input blocks
+  const S* inp;
+  int64_t T;
+};
+
+template <typename S, int NP, bool VEC, bool FWD, int JJ>
+__device__ __forceinline__ void zsum_steps(ZsumLane<S, NP, VEC>& st, int64_t tb) {
+  using W = typename Wide<S>::type;
+  using RC = RerunCfg<S, NP>;
+  if constexpr (JJ < RC::K) {
+    // forward: t = tb + JJ (tb a multiple of NP and of 4); adjoint: t = tb - JJ
+    // (tb is NP-1 mod NP and 3 mod 4)
+    constexpr int j = FWD ? JJ % NP : NP - 1 - JJ % NP;
+    constexpr int comp = FWD ? (JJ & 3) : 3 - (JJ & 3);
+    const int64_t t = FWD ? tb + JJ : tb - JJ;
+    W v;
+    if constexpr (VEC)
+      v = convert<W>(v4get<comp>(st.xq[(JJ % RC::DS) / 4]));
+    else
+      v = convert<W>(st.xs[JJ % RC::DS]);
+    if constexpr (VEC) {
+      if constexpr ((JJ & 3) == 3)
+        st.xq[(JJ % RC::DS) / 4] =
+            ld4(st.inp, FWD ? min(t - 3 + RC::DS, st.T - 4) : clamp64(t - RC::DS, 0, st.T - 4));
+    } else {
+      st.xs[JJ % RC::DS] = st.inp[FWD ? min(t + RC::DS, st.T - 1) : clamp64(t - RC::DS, 0, st.T - 1)];
+    }
+    if constexpr (FWD) {
+      // chunks 0..P-2 are whole: t < T throughout
+      const W y = v + st.p[j];
+      st.p[j] = Num<W>::zero();
+#pragma unroll
+      for (int i = 1; i <= NP; ++i) st.p[(j + i) & (NP - 1)] = Num<W>::fms(st.p[(j + i) & (NP - 1)], st.co[i - 1], y);
+      st.yh[j] = y;
+    } else {
+      // steps at or past T have g = 0 and leave the zero state as it is
+      const W lam = (t < st.T ? v : Num<W>::zero()) + st.p[j];
+      st.p[j] = Num<W>::zero();
+#pragma unroll
+      for (int i = 1; i <= NP; ++i) st.p[(j - i) & (NP - 1)] = Num<W>::fms(st.p[(j - i) & (NP - 1)], st.co[i - 1], lam);
+    }
+    pin_step();
+    zsum_steps<S, NP, VEC, FWD, JJ + 1>(st, tb);
+  }
+}
+
+template <typename S, int NP, bool FWD, bool VEC>
 __global__ __launch_bounds__(256) void ti_zsum(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
+  using RC = RerunCfg<S, NP>;
   const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t cps = a.P - 1;
   if (item >= a.B * cps) return;
   const int64_t b = item / cps;
   const int64_t c = item - b * cps + (FWD ? 0 : 1);
   const int64_t t0 = c * a.L;
-  W co[NP];
-  load_coefs<S, NP>(co, a, b, !FWD);
-  W ring[NP];
+  ZsumLane<S, NP, VEC> st;
+  load_coefs<S, NP>(st.co, a, b, !FWD);
 #pragma unroll
-  for (int r = 0; r < NP; ++r) ring[r] = Num<W>::zero();
-  const S* inp = a.x + b * a.T;
-  if constexpr (FWD) {
-    // chunks 0..P-2 are whole: t < T throughout
-    for (int jb = 0; jb < a.L; jb += NP) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        W acc0 = convert<W>(inp[t0 + jb + j]), acc1 = Num<W>::zero();
-#pragma unroll
-        for (int i = 1; i <= NP; i += 2) {
-          acc0 = Num<W>::fms(acc0, co[i - 1], ring[(j - i) & (NP - 1)]);
-          if (i < NP) acc1 = Num<W>::fms(acc1, co[i], ring[(j - i - 1) & (NP - 1)]);
-        }
-        ring[j] = acc0 + acc1;
-      }
-    }
-  } else {
-    // descending; steps at or past T have g = 0 and leave the zero state as it is
-    for (int jb = 0; jb < a.L; jb += NP) {
-#pragma unroll
-      for (int jj = 0; jj < NP; ++jj) {
-        const int j = NP - 1 - jj;
-        const int64_t t = t0 + a.L - 1 - jb - jj;
-        const W lam = (t < a.T ? convert<W>(inp[t]) : Num<W>::zero()) + ring[j];
-        ring[j] = Num<W>::zero();
-#pragma unroll
-        for (int i = 1; i <= NP; ++i) ring[(j - i) & (NP - 1)] = Num<W>::fms(ring[(j - i) & (NP - 1)], co[i - 1], lam);
-      }
-    }
+  for (int r = 0; r < NP; ++r) {
+    st.p[r] = Num<W>::zero();
+    st.yh[r] = Num<W>::zero();
   }
-  W* out = a.summ + item * NP;
+  st.inp = a.x + b * a.T;
+  st.T = a.T;
+  const int64_t tfirst = FWD ? t0 : t0 + a.L - 1;
+  if constexpr (VEC) {
 #pragma unroll
-  for (int r = 0; r < NP; ++r) out[r] = ring[NP - 1 - r];  // state at the chunk's far end
+    for (int q = 0; q < RC::DS / 4; ++q)
+      st.xq[q] = opaque_v(ld4(st.inp, FWD ? min(tfirst + 4 * q, a.T - 4) : clamp64(tfirst - 3 - 4 * q, 0, a.T - 4)));
+  } else {
+#pragma unroll
+    for (int k = 0; k < RC::DS; ++k)
+      st.xs[k] = opaque_v(st.inp[FWD ? min(tfirst + k, a.T - 1) : clamp64(tfirst - k, 0, a.T - 1)]);
+  }
+  for (int jb = 0; jb < a.L; jb += RC::K) zsum_steps<S, NP, VEC, FWD, 0>(st, FWD ? t0 + jb : tfirst - jb);
+  W* out = a.summ + item * NP;
+  // the state at the chunk's far end: y_{t0+L-1-r} (forward), the pending sums of
+  // positions t0-1-r (adjoint); both sit in slot NP-1-r
+#pragma unroll
+  for (int r = 0; r < NP; ++r) out[r] = FWD ? st.yh[NP - 1 - r] : st.p[NP - 1 - r];
 }
 
 template <typename S, int NP, bool FWD>
@@ -1004,31 +1134,54 @@ __global__ __launch_bounds__(64) void carry_ti(ChunkArgs<S> a) {
     for (int r = 0; r < NP; ++r) phi[r][lane] = ring[NP - 1 - r];
   }
   __syncthreads();
+  // Lanes NP..63 shadow lane NP-1 (same row, same values), so the chain's
+  // stores need no branch: a store under a divergent branch makes the wait-count
+  // pass drain the prefetch ring at every link.
+  const int rl = lane < NP ? lane : NP - 1;
   W s = Num<W>::zero();
-  if (FWD && lane < a.n && a.zi) s = convert<W>(a.zi[b * a.n + lane]);
+  if (FWD && rl < a.n && a.zi) s = convert<W>(a.zi[b * a.n + rl]);
   W* carry = a.carry + b * P * NP;
   const W* zs = a.summ + b * (P - 1) * NP;
-  for (int64_t step = 0; step < P; ++step) {
-    const int64_t c = FWD ? step : P - 1 - step;
-    if (lane < NP) {
-      carry[c * NP + lane] = s;
-      sv[lane] = s;
-    }
-    if (step == P - 1) break;
-    __syncthreads();
-    if (lane < NP) {
-      // forward chunk c is at slot c; adjoint chunk c (>= 1) at slot c-1
-      W acc0 = zs[(FWD ? c : c - 1) * NP + lane], acc1 = Num<W>::zero();
+  // Each lane keeps its row of the link matrix (Phi, or Phi^H for the adjoint)
+  // in registers, and the links' zero-state vectors are prefetched ZD links
+  // ahead into a register ring, by every lane at a clamped slot (forward link
+  // `step` reads chunk slot step, adjoint link `step` chunk P-1-step at slot
+  // P-2-step), so that no load sits under a branch. A slot is read after the
+  // link's products and reloaded after that read, so old and new values never
+  // overlap in its register; the barriers fence LDS only.
+  W mrow[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) mrow[k] = FWD ? phi[rl][k] : Num<W>::conj(phi[k][rl]);
+  constexpr int ZD = 8;
+  auto zaddr = [&](int64_t step) -> const W* {
+    const int64_t q = clamp64(FWD ? step : P - 2 - step, 0, P - 2);
+    return zs + q * NP + rl;
+  };
+  W zr[ZD];
+  if (P > 1) {
+#pragma unroll
+    for (int d = 0; d < ZD; ++d) zr[d] = opaque_v(*zaddr(d));
+  }
+  for (int64_t sb = 0; sb < P; sb += ZD) {
+#pragma unroll
+    for (int d = 0; d < ZD; ++d) {
+      const int64_t step = sb + d;
+      if (step >= P) break;
+      const int64_t c = FWD ? step : P - 1 - step;
+      carry[c * NP + rl] = s;
+      sv[rl] = s;
+      if (step == P - 1) break;
+      lds_sync();
+      W acc0 = Num<W>::zero(), acc1 = Num<W>::zero();
 #pragma unroll
       for (int k = 0; k < NP; k += 2) {
-        const W m0 = FWD ? phi[lane][k] : Num<W>::conj(phi[k][lane]);
-        const W m1 = FWD ? phi[lane][k + 1 < NP ? k + 1 : k] : Num<W>::conj(phi[k + 1 < NP ? k + 1 : k][lane]);
-        acc0 = Num<W>::fma(acc0, m0, sv[k]);
-        if (k + 1 < NP) acc1 = Num<W>::fma(acc1, m1, sv[k + 1]);
+        acc0 = Num<W>::fma(acc0, mrow[k], sv[k]);
+        if (k + 1 < NP) acc1 = Num<W>::fma(acc1, mrow[k + 1], sv[k + 1]);
       }
-      s = acc0 + acc1;
+      s = (acc0 + acc1) + zr[d];
+      zr[d] = *zaddr(step + ZD);
+      lds_sync();
     }
-    __syncthreads();
   }
 }
 
@@ -1044,15 +1197,38 @@ int64_t chunk_count(const ChunkArgs<S>& a) {
   return a.rg ? a.G : a.B * a.P;
 }
 
+// float4 streams of the time-invariant kernels: float32 data, whole 4-sample
+// blocks (T a multiple of 4; chunk lengths are multiples of 16) and 16-byte
+// aligned streams (a null output counts as aligned).
+template <typename S>
+constexpr bool kF32 = std::is_same<S, float>::value;  // the float4 kernels exist for float32 only
+template <typename S>
+bool ti_vec(const ChunkArgs<S>& a, const void* in, const void* out) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if constexpr (std::is_same<S, float>::value)
+    return a.T % 4 == 0 && a.L % 4 == 0 && al(in) && (out == nullptr || al(out));
+  return false;
+}
+
 template <typename S, int NP>
 int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
   if (a.a_tstride == 0 && !a.rg) {
     const int64_t nz = a.B * (a.P - 1);
-    if (nz > 0) hipLaunchKernelGGL((ti_zsum<S, NP, true>), dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, a);
+    const dim3 gz((unsigned)((nz + 255) / 256));
+    if (nz > 0) {
+      if (ti_vec(a, a.x, nullptr))
+        hipLaunchKernelGGL((ti_zsum<S, NP, true, kF32<S>>), gz, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((ti_zsum<S, NP, true, false>), gz, dim3(256), 0, st, a);
+    }
     hipLaunchKernelGGL((carry_ti<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
-    hipLaunchKernelGGL((fwd_rerun_ti<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+    const dim3 g((unsigned)((nch + 255) / 256));
+    if (ti_vec(a, a.x, a.y))
+      hipLaunchKernelGGL((fwd_rerun_ti<S, NP, kF32<S>>), g, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((fwd_rerun_ti<S, NP, false>), g, dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   if (a.P > 1) {
@@ -1079,18 +1255,32 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
   if (a.a_tstride == 0 && !a.rg) {
     const int64_t nz = a.B * (a.P - 1);
-    if (nz > 0) hipLaunchKernelGGL((ti_zsum<S, NP, false>), dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, a);
+    const dim3 gz((unsigned)((nz + 255) / 256));
+    if (nz > 0) {
+      if (ti_vec(a, a.x, nullptr))
+        hipLaunchKernelGGL((ti_zsum<S, NP, false, kF32<S>>), gz, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((ti_zsum<S, NP, false, false>), gz, dim3(256), 0, st, a);
+    }
     hipLaunchKernelGGL((carry_ti<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
     const dim3 g((unsigned)((nch + 255) / 256));
+    const bool vec = ti_vec(a, a.x, a.dx);
     // a full per-sample dA of a time-invariant filter takes the per-step row
     // kernel, which reads the one row at stride 0
-    if (a.dapart)
-      hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL>), g, dim3(256), 0, st, a);
-    else if (a.dA)
+    if (a.dapart) {
+      if (vec)
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL, kF32<S>>), g, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL, false>), g, dim3(256), 0, st, a);
+    } else if (a.dA) {
       hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(256), 0, st, a);
-    else
-      hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE>), g, dim3(256), 0, st, a);
+    } else {
+      if (vec)
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE, kF32<S>>), g, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE, false>), g, dim3(256), 0, st, a);
+    }
     if (a.dapart) {
       const int64_t nd = a.B * a.n;
       hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);

@@ -57,6 +57,42 @@ __device__ __forceinline__ double adj_row_step(double (&p)[NP], double a) {
   return lam;
 }
 
+// The same step with the coefficients of four consecutive steps held as float4
+// columns: for step J, lane 4*(J & 3) + (k >> 2) of the row holds A[t, k] in
+// component k & 3 of d (its float4 converted to float64). A 16-lane row then
+// loads a 4-step group of rows (256 contiguous bytes) with one float4 per lane.
+template <int J>
+__device__ __forceinline__ double adj_row_step4(double (&p)[NP], const double (&d)[4]) {
+  constexpr int L0 = 4 * (J & 3);
+  const double lam = p[J];
+  double fresh = 0.0;
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, -%16, %20 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%17, %20 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, -%18, %20 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, -%19, %20 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, -%16, %20 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, -%17, %20 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, -%18, %20 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %7, -%19, %20 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %8, -%16, %20 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %9, -%17, %20 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %10, -%18, %20 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %11, -%19, %20 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %12, -%16, %20 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %13, -%17, %20 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %14, -%18, %20 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %15, -%19, %20 row_newbcast:%24 row_mask:0xf bank_mask:0xf"
+      : "+v"(p[(J - 1) & 15]), "+v"(p[(J - 2) & 15]), "+v"(p[(J - 3) & 15]), "+v"(p[(J - 4) & 15]),
+        "+v"(p[(J - 5) & 15]), "+v"(p[(J - 6) & 15]), "+v"(p[(J - 7) & 15]), "+v"(p[(J - 8) & 15]),
+        "+v"(p[(J - 9) & 15]), "+v"(p[(J - 10) & 15]), "+v"(p[(J - 11) & 15]), "+v"(p[(J - 12) & 15]),
+        "+v"(p[(J - 13) & 15]), "+v"(p[(J - 14) & 15]), "+v"(p[(J - 15) & 15]), "+v"(fresh)
+      : "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]), "v"(lam), "i"(L0), "i"(L0 + 1), "i"(L0 + 2),
+        "i"(L0 + 3));
+  p[J] = fresh;
+  return lam;
+}
+
 // acc += v(lane K of the row) * w: a row broadcast of a per-lane value. The
 // s_nop 1 covers a v produced by the VALU right before (a DPP source needs two
 // wait states).
