@@ -311,6 +311,28 @@ def _offset_view(a):
     return v
 
 
+@pytest.mark.parametrize("N", [2, 4, 8])
+@pytest.mark.parametrize("T,chunk", [(64, 16), (100, 32), (1000, 0), (4096, 48), (2048, 0)])
+def test_float4_stream_reruns(N, T, chunk):
+    """Float32 time-varying orders 2, 4 and 8 with T a multiple of 4 take the
+    float4-stream reruns (chunk 0 reads zi through its y window); the same data
+    from views that start 4 bytes into their allocation take the per-sample
+    reruns. Both meet the FP64 oracle."""
+    rng = np.random.default_rng(N * 1000 + T + chunk)
+    B = 3
+    x, A, zi = stable_inputs(rng, B, T, N, np.float32)
+    g = stable_inputs(rng, B, T, 1, np.float32)[0]
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    ref = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    for aligned in (True, False):
+        view = dev if aligned else _offset_view
+        y = ops().lpc(view(x), dev(A), dev(zi), 1, chunk)
+        assert rel(host(y), y64) <= 1e-5, aligned
+        outs = ops().lpc_backward(view(g), dev(A), dev(zi), y, True, True, True, 1, chunk)
+        for name, o, r in zip(("dx", "dA", "dzi"), outs, ref):
+            assert rel(host(o), r) <= 1e-5, (aligned, name, rel(host(o), r))
+
+
 def test_time_invariant_unaligned_views():
     """Time-invariant float32 streams that start 4 bytes into their allocation
     take the per-sample loads; results match the aligned call bit for bit."""

@@ -968,6 +968,254 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// Reruns with float4 streams, time-varying float32 filters of order 2, 4 or 8
+// (n == NP) in a uniform batch with T a multiple of 4 and 16-byte aligned
+// streams. One lane per chunk, as in fwd_rerun / bwd_rerun, but every stream
+// moves in 4-step blocks: the block's NP coefficient rows are NP float4, and x,
+// g, y, dx are one float4 each. The blocks are prefetched DS steps ahead in
+// rings whose slots are consumed before they are reloaded, all loads
+// unconditional at clamped block starts. A lane's 4-byte accesses are L2
+// requests of their own; the per-step kernels issue NP + 2 (forward) or 2NP + 4
+// (adjoint) of them per sample, these kernels about a quarter.
+template <int NP>
+struct Tv4 {
+  static constexpr int DS = NP <= 4 ? 16 : 8;  // prefetch distance in steps
+  static constexpr int QB = DS / 4;            // ring slots (4-step blocks)
+  static constexpr int K = 16;                 // steps per unrolled block
+};
+
+// The 4*M floats of M float4 as one array (indices fold to registers once the
+// callers' loops are unrolled).
+template <int M>
+__device__ __forceinline__ void flat4(const float4 (&v)[M], float (&f)[4 * M]) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    f[4 * k] = v[k].x;
+    f[4 * k + 1] = v[k].y;
+    f[4 * k + 2] = v[k].z;
+    f[4 * k + 3] = v[k].w;
+  }
+}
+
+template <int NP>
+struct FwdLaneTv4 {
+  using C = Tv4<NP>;
+  double ring[NP];            // y_u in slot u mod NP
+  float4 aq[C::QB][NP];       // rows of a 4-step block: 4*NP floats = NP float4
+  float4 xq[C::QB];
+  float4 yq;
+  const float* xb;
+  const float* Ab;
+  float* yb;
+  int64_t T, tend;
+};
+
+template <int NP, int JJ>
+__device__ __forceinline__ void fwd_steps_tv4(FwdLaneTv4<NP>& st, int64_t tb) {
+  using C = Tv4<NP>;
+  if constexpr (JJ < C::K) {
+    constexpr int j = JJ % NP;
+    constexpr int q = (JJ % C::DS) / 4;
+    constexpr int r = JJ & 3;
+    const int64_t t = tb + JJ;
+    float cf[4 * NP];
+    flat4<NP>(st.aq[q], cf);
+    double acc = (double)v4get<r>(st.xq[q]);
+#pragma unroll
+    for (int i = NP; i >= 1; --i) acc -= (double)cf[r * NP + i - 1] * st.ring[(j - i) & (NP - 1)];
+    if constexpr (r == 3) {
+      // the block's last step: its slots take the block DS steps ahead
+      const int64_t u = min(t - 3 + C::DS, st.T - 4);
+      const float4* ap = reinterpret_cast<const float4*>(st.Ab + u * NP);
+#pragma unroll
+      for (int k = 0; k < NP; ++k) st.aq[q][k] = ap[k];
+      st.xq[q] = ld4(st.xb, u);
+    }
+    pin_step();
+    st.ring[j] = acc;
+    v4set<r>(st.yq, (float)acc);
+    if constexpr (r == 3)
+      if (t < st.tend) *reinterpret_cast<float4*>(st.yb + t - 3) = st.yq;
+    fwd_steps_tv4<NP, JJ + 1>(st, tb);
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void fwd_rerun_tv4(ChunkArgs<float> a) {
+  using C = Tv4<NP>;
+  const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
+  const int64_t t0 = c * a.L;
+  const int64_t rem = a.T - t0;
+  const int len = rem < a.L ? (int)rem : a.L;
+  FwdLaneTv4<NP> st;
+  st.xb = a.x + b * a.T;
+  st.Ab = a.A + b * a.a_bstride;
+  st.yb = a.y + b * a.T;
+  st.T = a.T;
+  st.tend = t0 + len;
+#pragma unroll
+  for (int q = 0; q < C::QB; ++q) {
+    const int64_t u = min(t0 + 4 * q, a.T - 4);
+    const float4* ap = reinterpret_cast<const float4*>(st.Ab + u * NP);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) st.aq[q][k] = opaque_v(ap[k]);
+    st.xq[q] = opaque_v(ld4(st.xb, u));
+  }
+  const double* cr = a.carry + cid * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = opaque_v(cr[r]);
+  for (int jb = 0; jb < len; jb += C::K) fwd_steps_tv4<NP, 0>(st, t0 + jb);
+}
+
+template <int NP>
+struct BwdLaneTv4 {
+  using C = Tv4<NP>;
+  double p[NP];          // pending sums, slot u mod NP for position u
+  float yr[NP];          // y window: slot u mod NP holds y_u for u in [t-NP, t-1]
+  float zr[NP];          // y_{-1-k} = zi[k] (zero past the order or without zi)
+  float4 aq[C::QB][NP];  // coefficient rows of 4-step blocks
+  float4 gq[C::QB];
+  float4 yq[C::QB];      // y blocks of the window input (positions t-1-NP)
+  float4 dq;             // lambda of the current block
+  float daf[4 * NP];     // dA rows of the current block
+  const float* gb;
+  const float* Ab;
+  const float* yb;
+  float* dxb;
+  float* dAb;
+  int64_t T;
+};
+
+// y[u .. u+3] as the window reads it: y_{-1-k} = zi[k] below 0 (chunk 0 only),
+// clamped into the signal above (values past T meet only steps past T).
+template <int NP>
+__device__ __forceinline__ float4 yblock(const BwdLaneTv4<NP>& st, int64_t u) {
+  float4 v = ld4(st.yb, clamp64(u, 0, st.T - 4));
+  float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int64_t k = -(u + m) - 1;  // zi index when u + m < 0
+    float z = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < NP; ++kk) z = (k == kk) ? st.zr[kk] : z;
+    c[m] = (u + m < 0) ? z : c[m];
+  }
+  return make_float4(c[0], c[1], c[2], c[3]);
+}
+
+constexpr int floordiv4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+constexpr int posmod(int v, int m) { return ((v % m) + m) % m; }
+
+// Steps t = tb - JJ (tb is 3 mod 4 and NP-1 mod NP). Block slot q holds
+// positions [u, u+3] with u = t - e, e = 3 - JJ % 4. The window input
+// y_{t-1-NP} sits at element (2 - JJ - NP) mod 4 of y slot
+// (QB - 1 + floor((2 - JJ - NP) / 4)) mod QB (the slot index of the block that
+// holds it: tb = 4m + 3 with m = QB - 1 mod QB at every unrolled block).
+template <int NP, int JJ>
+__device__ __forceinline__ void bwd_steps_tv4(BwdLaneTv4<NP>& st, int64_t tb) {
+  using C = Tv4<NP>;
+  if constexpr (JJ < C::K) {
+    constexpr int j = NP - 1 - JJ % NP;
+    constexpr int q = (JJ % C::DS) / 4;
+    constexpr int e = 3 - (JJ & 3);
+    constexpr int ye = posmod(2 - JJ - NP, 4);
+    constexpr int yslot = posmod(C::QB - 1 + floordiv4(2 - JJ - NP), C::QB);
+    const int64_t t = tb - JJ;
+    const bool live = t < st.T;
+    const double lam0 = (double)v4get<e>(st.gq[q]) + st.p[j];
+    const double lam = live ? lam0 : 0.0;
+    st.p[j] = live ? 0.0 : st.p[j];
+    float cf[4 * NP];
+    flat4<NP>(st.aq[q], cf);
+#pragma unroll
+    for (int i = 1; i <= NP; ++i) st.p[(j - i) & (NP - 1)] -= (double)cf[e * NP + i - 1] * lam;
+    v4set<e>(st.dq, (float)lam);
+#pragma unroll
+    for (int i = 1; i <= NP; ++i) st.daf[e * NP + i - 1] = (float)(-(lam * (double)st.yr[(j - i) & (NP - 1)]));
+    // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
+    st.yr[(j - 1) & (NP - 1)] = v4get<ye>(st.yq[yslot]);
+    if constexpr (ye == 0) st.yq[yslot] = yblock<NP>(st, t - 1 - NP - C::DS);
+    if constexpr (e == 0) {
+      // u = t: the block is complete; its slots take the block DS steps below
+      const int64_t u = clamp64(t - C::DS, 0, st.T - 4);
+      const float4* ap = reinterpret_cast<const float4*>(st.Ab + u * NP);
+#pragma unroll
+      for (int k = 0; k < NP; ++k) st.aq[q][k] = ap[k];
+      st.gq[q] = ld4(st.gb, u);
+      // T is a multiple of 4: the block is all live or all past T
+      if (live) {
+        if (st.dxb) *reinterpret_cast<float4*>(st.dxb + t) = st.dq;
+        if (st.dAb) {
+          float4* dp = reinterpret_cast<float4*>(st.dAb + t * NP);
+#pragma unroll
+          for (int k = 0; k < NP; ++k)
+            dp[k] = make_float4(st.daf[4 * k], st.daf[4 * k + 1], st.daf[4 * k + 2], st.daf[4 * k + 3]);
+        }
+      }
+    }
+    pin_step();
+    bwd_steps_tv4<NP, JJ + 1>(st, tb);
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void bwd_rerun_tv4(ChunkArgs<float> a) {
+  using C = Tv4<NP>;
+  const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
+  const int64_t t0 = c * a.L;
+  BwdLaneTv4<NP> st;
+  st.gb = a.x + b * a.T;  // upstream gradient
+  st.Ab = a.A + b * a.a_bstride;
+  st.yb = a.y + b * a.T;
+  st.dxb = a.dx ? a.dx + b * a.T : nullptr;
+  st.dAb = a.dA ? a.dA + b * a.T * NP : nullptr;
+  st.T = a.T;
+  const float* zib = a.zi ? a.zi + b * a.n : nullptr;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) st.zr[k] = zib ? zib[k] : 0.f;
+  const int64_t tstart = t0 + a.L - 1;  // 3 mod 4 and NP-1 mod NP
+#pragma unroll
+  for (int q = 0; q < C::QB; ++q) {
+    const int64_t u = clamp64(tstart - 3 - 4 * q, 0, a.T - 4);
+    const float4* ap = reinterpret_cast<const float4*>(st.Ab + u * NP);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) st.aq[q][k] = opaque_v(ap[k]);
+    st.gq[q] = opaque_v(ld4(st.gb, u));
+  }
+  // the y blocks that hold positions tstart-1-NP and the DS - 4 below it
+  {
+    const int64_t pf = tstart - 1 - NP;
+    const int64_t u0 = pf - posmod(2 - NP, 4);  // block start of pf
+    // (the slot of block u0 - 4k is a constant once the loop is unrolled)
+#pragma unroll
+    for (int k = 0; k < C::QB; ++k)
+      st.yq[posmod(C::QB - 1 + floordiv4(2 - NP) - k, C::QB)] = opaque_v(yblock<NP>(st, u0 - 4 * k));
+  }
+  // window y_{tstart-1} .. y_{tstart-NP}
+#pragma unroll
+  for (int i = 1; i <= NP; ++i) {
+    const int64_t u = tstart - i;
+    const float v = *yw_addr(st.yb, zib, a.n, a.T, u);
+    st.yr[(NP - 1 - i + NP) & (NP - 1)] = yw_ok(zib, a.n, a.T, u) ? v : 0.f;
+  }
+  const double* cr = a.carry + cid * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) st.p[NP - 1 - r] = opaque_v(cr[r]);
+  for (int jb = 0; jb < a.L; jb += C::K) bwd_steps_tv4<NP, 0>(st, tstart - jb);
+  if (c == 0 && a.dzi) {
+    // pending sums for positions -1-r sit in slots NP-1-r
+#pragma unroll
+    for (int r = 0; r < NP; ++r) a.dzi[b * a.n + r] = (float)st.p[NP - 1 - r];
+  }
+}
+
 // Time-invariant da: deterministic sum of the per-chunk partials (wide type).
 template <typename S, int NP>
 __global__ __launch_bounds__(256) void ti_da_reduce(ChunkArgs<S> a) {
@@ -1210,6 +1458,18 @@ bool ti_vec(const ChunkArgs<S>& a, const void* in, const void* out) {
   return false;
 }
 
+// The float4-stream reruns of time-varying float32 filters of order 2, 4, 8:
+// uniform batch, n == NP, contiguous rows, T a multiple of 4, 16-byte aligned
+// streams (x or g, y, A, and the outputs that are present).
+template <typename S, int NP>
+constexpr bool kTv4 = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8);
+template <typename S>
+bool tv4_ok(const ChunkArgs<S>& a, int np, const void* y, const void* dx, const void* dA) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  return !a.rg && !a.neg && a.n == np && a.a_tstride == a.n && a.dapart == nullptr && a.T % 4 == 0 && a.L % 16 == 0 &&
+         al(a.x) && al(a.A) && al(y) && (!dx || al(dx)) && (!dA || al(dA));
+}
+
 template <typename S, int NP>
 int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
@@ -1246,7 +1506,14 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_pipe<S, NP, true>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
-  hipLaunchKernelGGL((fwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+  const dim3 g((unsigned)((nch + 255) / 256));
+  if constexpr (kTv4<S, NP>) {
+    if (tv4_ok(a, NP, a.y, nullptr, nullptr)) {
+      hipLaunchKernelGGL((fwd_rerun_tv4<NP>), g, dim3(256), 0, st, a);
+      return hipGetLastError() == hipSuccess ? 0 : -4;
+    }
+  }
+  hipLaunchKernelGGL((fwd_rerun<S, NP>), g, dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
@@ -1301,7 +1568,15 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_pipe<S, NP, false>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
-  hipLaunchKernelGGL((bwd_rerun<S, NP>), dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, st, a);
+  const dim3 g((unsigned)((nch + 255) / 256));
+  bool tv4 = false;
+  if constexpr (kTv4<S, NP>) {
+    if (tv4_ok(a, NP, a.y, a.dx, a.dA)) {
+      hipLaunchKernelGGL((bwd_rerun_tv4<NP>), g, dim3(256), 0, st, a);
+      tv4 = true;
+    }
+  }
+  if (!tv4) hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(256), 0, st, a);
   if (a.dapart) {
     const int64_t nd = a.B * a.n;
     hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);
