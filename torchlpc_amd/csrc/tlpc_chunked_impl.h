@@ -94,29 +94,56 @@ __device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP,
 
 // Staging helper: copy the A rows and the per-step input of the G items for the
 // time block [t0, t0 + TB) into LDS; rows beyond the signal end and lags >= n
-// are zero.
+// are zero. Every thread first issues all of its loads, unconditionally (an
+// element that must read as zero loads a valid address and is selected away),
+// then writes LDS: a load under a branch is waited for at once, which had
+// serialised the thread's loads. The kernels run C::THREADS threads.
 template <typename S, int NP, int LANES>
 __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide<S>::type* sA,
                                             typename Wide<S>::type* sX, const S* inp,
                                             const ItemTab<S, NP, LANES>& tab, int t0) {
   using C = ChunkCfg<S, NP, LANES>;
   using W = typename Wide<S>::type;
-  const int nthr = blockDim.x;
-  for (int idx = threadIdx.x; idx < C::G * C::TB * NP; idx += nthr) {
-    const int gg = idx / (C::TB * NP);
-    const int r = idx - gg * (C::TB * NP);
+  constexpr int NT = C::THREADS;
+  constexpr int NA = C::G * C::TB * NP;
+  constexpr int NX = C::G * C::TB;
+  constexpr int ITA = (NA + NT - 1) / NT;
+  constexpr int ITX = (NX + NT - 1) / NT;
+  S va[ITA], vx[ITX];
+  bool oka[ITA], okx[ITX];
+#pragma unroll
+  for (int it = 0; it < ITA; ++it) {
+    const int idx = it * NT + (int)threadIdx.x;
+    const int gg = min(idx, NA - 1) / (C::TB * NP);
+    const int r = min(idx, NA - 1) - gg * (C::TB * NP);
     const int tb = r / NP;
     const int i = r - tb * NP;
-    W v = Num<W>::zero();
-    if (t0 + tb < tab.rem[gg] && i < a.n) v = convert<W>(coef(a, a.A[tab.a0[gg] + (t0 + tb) * tab.at[gg] + i]));
-    sA[gg * C::ASTRIDE + tb * NP + i] = v;
+    const bool ok = idx < NA && t0 + tb < tab.rem[gg] && i < a.n;
+    va[it] = a.A[ok ? tab.a0[gg] + (t0 + tb) * tab.at[gg] + i : 0];
+    oka[it] = ok;
   }
-  for (int idx = threadIdx.x; idx < C::G * C::TB; idx += nthr) {
-    const int gg = idx / C::TB;
-    const int tb = idx - gg * C::TB;
-    W v = Num<W>::zero();
-    if (t0 + tb < tab.rem[gg]) v = convert<W>(inp[tab.x0[gg] + t0 + tb]);
-    sX[gg * C::TB + tb] = v;
+#pragma unroll
+  for (int it = 0; it < ITX; ++it) {
+    const int idx = it * NT + (int)threadIdx.x;
+    const int gg = min(idx, NX - 1) / C::TB;
+    const int tb = min(idx, NX - 1) - gg * C::TB;
+    const bool ok = idx < NX && t0 + tb < tab.rem[gg];
+    vx[it] = inp[ok ? tab.x0[gg] + t0 + tb : 0];
+    okx[it] = ok;
+  }
+#pragma unroll
+  for (int it = 0; it < ITA; ++it) {
+    const int idx = it * NT + (int)threadIdx.x;
+    if (idx < NA) {
+      const int gg = idx / (C::TB * NP);
+      const int r = idx - gg * (C::TB * NP);
+      sA[gg * C::ASTRIDE + r] = oka[it] ? convert<W>(coef(a, va[it])) : Num<W>::zero();
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITX; ++it) {
+    const int idx = it * NT + (int)threadIdx.x;
+    if (idx < NX) sX[idx] = okx[it] ? convert<W>(vx[it]) : Num<W>::zero();
   }
 }
 
@@ -416,12 +443,19 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
   auto link = [&](int64_t l) -> const W* {
     return a.summ + summ_slot(a, b, sp, FWD ? l : P - 1 - l, FWD) * ChunkCfg<S, NP>::SUMM;
   };
+  // A thread walks its run in batches of UR links whose loads are issued
+  // together (at clamped link indices, so unconditionally) before the batch's
+  // arithmetic: the chain then waits for memory once per batch, not per link.
+  constexpr int UR = NP <= 2 ? 8 : 4;
   Affine<W, NP> f;
   affine_identity(f);
-  for (int64_t l = lo; l < hi; ++l) {
-    Affine<W, NP> m;
-    load_affine(m, link(l));
-    f = affine_compose(m, f);
+  for (int64_t l = lo; l < hi; l += UR) {
+    Affine<W, NP> m[UR];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) load_affine(m[u], link(l + u < hi ? l + u : hi - 1));
+#pragma unroll
+    for (int u = 0; u < UR; ++u)
+      if (l + u < hi) f = affine_compose(m[u], f);
   }
   sh[tid] = f;
   __syncthreads();
@@ -456,21 +490,28 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
     for (int r = 0; r < NP; ++r) s[r] = t[r];
   }
   W* carry = a.carry + sp.cbase * NP;
-  for (int64_t l = lo; l < hi; ++l) {
-    const int64_t c = FWD ? l : P - 1 - l;
+  for (int64_t l = lo; l < hi; l += UR) {
+    Affine<W, NP> m[UR];
 #pragma unroll
-    for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
-    const W* sm = link(l);
-    W t[NP];
+    for (int u = 0; u < UR; ++u) load_affine(m[u], link(l + u < hi ? l + u : hi - 1));
 #pragma unroll
-    for (int r = 0; r < NP; ++r) {
-      W acc = sm[NP * NP + r];
+    for (int u = 0; u < UR; ++u) {
+      if (l + u < hi) {
+        const int64_t c = FWD ? l + u : P - 1 - (l + u);
 #pragma unroll
-      for (int k = 0; k < NP; ++k) acc = Num<W>::fma(acc, sm[r * NP + k], s[k]);
-      t[r] = acc;
+        for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
+        W t[NP];
+#pragma unroll
+        for (int r = 0; r < NP; ++r) {
+          W acc = m[u].v[r];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) acc = Num<W>::fma(acc, m[u].M[r][k], s[k]);
+          t[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < NP; ++r) s[r] = t[r];
+      }
     }
-#pragma unroll
-    for (int r = 0; r < NP; ++r) s[r] = t[r];
   }
   // the thread owning the last link (thread 0 when there is none) writes the final carry
   if ((lo < hi && hi == links) || (links == 0 && tid == 0)) {
