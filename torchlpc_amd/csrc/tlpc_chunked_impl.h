@@ -529,7 +529,27 @@ constexpr bool blocked_fits() {
 }
 
 // ----------------------------------------------------------------------------
-// Row loader: the NP coefficients of one time step, zero beyond the order.
+// Row loader: the NP coefficients of one time step, zero beyond the order. It
+// reads every slot at a valid address (lag min(i, n-1)) and selects
+// zero afterwards, so that no load depends on a branch.
+// The zeroing is a bit mask, not a select: the code generator turns a select
+// of a loaded value into a branch and sinks the load into it.
+__device__ __forceinline__ float keep_bits(float v, bool k) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & (k ? 0xffffffffu : 0u));
+}
+__device__ __forceinline__ double keep_bits(double v, bool k) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & (k ? ~0ull : 0ull));
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> keep_bits(Cx<R> v, bool k) {
+  return {keep_bits(v.re, k), keep_bits(v.im, k)};
+}
+template <typename S, int NP>
+__device__ __forceinline__ void load_row_u(S (&row)[NP], const ChunkArgs<S>& a, const S* p) {
+#pragma unroll
+  for (int i = 0; i < NP; ++i) row[i] = keep_bits(coef(a, p[i < a.n ? i : a.n - 1]), i < a.n);
+}
+// The plain form, for steps that sit under a branch of their own.
 template <typename S, int NP>
 __device__ __forceinline__ void load_row(S (&row)[NP], const ChunkArgs<S>& a, const S* p) {
   if (a.n == NP) {
@@ -852,11 +872,19 @@ __global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
 }
 
 // Phase 3 (reruns), general: one lane per chunk, loading each step's row and
-// sample when the step needs it. Time-varying rows do not fit a prefetch ring in
-// the lane's registers at the higher orders, and at the low orders the lane's
-// 4-byte accesses (one L2 request each) bound the kernel whether prefetched or
-// not (measured). Ragged batches and a full dA of time-invariant filters take
-// these kernels too.
+// sample as the step comes. Time-varying rows do not fit a prefetch ring in the
+// lane's registers at the higher orders. Ragged batches, other dtypes, orders
+// and alignments, and a full dA of time-invariant filters take these kernels.
+//
+// Up to order 4 (BL) the steps are branch-free: loads are unconditional, at
+// times clamped into the chunk, so the compiler may issue a step's loads ahead
+// of the previous step's arithmetic (a load under a branch is waited for at
+// the branch's end); steps past the end compute on clamped data, and their
+// stores go to the lane's own carry slot (already read, and rewritten by the
+// next carry pass) instead of being skipped (a store under a divergent branch
+// also makes the wait-count pass drain the loads in flight). At orders 8 to 32
+// that form measured slower in the ragged batch (more registers, the row's
+// loads interleaved with its products), so those steps keep the branch.
 //
 // Forward: one lane per chunk re-filters from its carry and writes y.
 // The recurrence runs in the wide type (float64 for float32 data), so the only
@@ -879,10 +907,23 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   const S* xb = a.x + sp.xo;
   const S* Ab = a.A + sp.ao;
   S* yb = a.y + sp.xo;
+  S* sink = reinterpret_cast<S*>(a.carry + cid * NP);
+  constexpr bool BL = NP <= 4;
   for (int jb = 0; jb < len; jb += NP) {
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
-      if (jb + j < len) {
+      if constexpr (BL) {
+        const bool live = jb + j < len;
+        const int64_t t = t0 + (live ? jb + j : len - 1);
+        S row[NP];
+        load_row_u<S, NP>(row, a, Ab + t * sp.at);
+        W acc = convert<W>(xb[t]);
+#pragma unroll
+        for (int i = NP; i >= 1; --i)
+          acc = Num<W>::fms(acc, convert<W>(row[i - 1]), ring[(j - i) & (NP - 1)]);
+        ring[j] = acc;
+        *(live ? yb + t : sink) = convert<S>(acc);
+      } else if (jb + j < len) {
         const int64_t t = t0 + jb + j;
         S row[NP];
         load_row<S, NP>(row, a, Ab + t * sp.at);
@@ -918,9 +959,44 @@ struct BwdLane {
   const S* zib;
   S* dxb;
   S* dAb;
+  S* sink;  // the lane's carry slot: destination of the stores of steps past T
   int64_t at, T;
   bool need_da;
 };
+
+// One adjoint step under a branch (orders 8 to 32; see the note above fwd_rerun).
+template <typename S, int NP, int JJ>
+__device__ __forceinline__ void bwd_step_branchy(const ChunkArgs<S>& a, BwdLane<S, NP>& st, int64_t t) {
+  using W = typename Wide<S>::type;
+  constexpr int j = NP - 1 - JJ;
+  if (t < st.T) {
+    S row[NP];
+    load_row<S, NP>(row, a, st.Ab + t * st.at);
+    const W lam = convert<W>(st.gb[t]) + st.ring[j];
+    st.ring[j] = Num<W>::zero();
+#pragma unroll
+    for (int i = 1; i <= NP; ++i)
+      st.ring[(j - i) & (NP - 1)] =
+          Num<W>::fms(st.ring[(j - i) & (NP - 1)], Num<W>::conj(convert<W>(row[i - 1])), lam);
+    if (st.dxb) st.dxb[t] = convert<S>(lam);
+    if (st.need_da) {
+      if (st.dAb) {
+        S* drow = st.dAb + t * a.n;
+#pragma unroll
+        for (int i = 1; i <= NP; ++i)
+          if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
+      } else {
+#pragma unroll
+        for (int i = 1; i <= NP; ++i)
+          st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
+      }
+    }
+  }
+  if (st.need_da) {
+    // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
+    st.yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval_at(st.yb, st.zib, a.n, st.T, t - 1 - NP));
+  }
+}
 
 // Steps t = tb - JJ for JJ = 0..NP-1 (t mod NP = NP-1-JJ), unrolled by template
 // recursion so that every ring index is a compile-time constant (a runtime index
@@ -931,32 +1007,38 @@ __device__ __forceinline__ void bwd_steps(const ChunkArgs<S>& a, BwdLane<S, NP>&
   if constexpr (JJ < NP) {
     constexpr int j = NP - 1 - JJ;
     const int64_t t = tb - JJ;
-    if (t < st.T) {
+    if constexpr (NP > 4) {
+      bwd_step_branchy<S, NP, JJ>(a, st, t);
+    } else {
+      // steps past the end of the signal (the last chunk's tail) have lambda = 0
+      // and leave the pending sums as they are
+      const bool live = t < st.T;
+      const int64_t tc = live ? t : st.T - 1;
       S row[NP];
-      load_row<S, NP>(row, a, st.Ab + t * st.at);
-      const W lam = convert<W>(st.gb[t]) + st.ring[j];
-      st.ring[j] = Num<W>::zero();
+      load_row_u<S, NP>(row, a, st.Ab + tc * st.at);
+      const W lam0 = convert<W>(st.gb[tc]) + st.ring[j];
+      const W lam = live ? lam0 : Num<W>::zero();
+      st.ring[j] = live ? Num<W>::zero() : st.ring[j];
 #pragma unroll
       for (int i = 1; i <= NP; ++i)
         st.ring[(j - i) & (NP - 1)] =
             Num<W>::fms(st.ring[(j - i) & (NP - 1)], Num<W>::conj(convert<W>(row[i - 1])), lam);
-      if (st.dxb) st.dxb[t] = convert<S>(lam);
-      if (st.need_da) {
-        if (st.dAb) {
-          S* drow = st.dAb + t * a.n;
+      if (st.dxb) *(live ? st.dxb + t : st.sink) = convert<S>(lam);
+      if (st.dAb) {
+        S* drow = live ? st.dAb + t * a.n : st.sink;
 #pragma unroll
-          for (int i = 1; i <= NP; ++i)
-            if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
-        } else {
+        for (int i = 1; i <= NP; ++i)
+          if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
+      } else if (st.need_da) {
 #pragma unroll
-          for (int i = 1; i <= NP; ++i)
-            st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
-        }
+        for (int i = 1; i <= NP; ++i)
+          st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
       }
-    }
-    if (st.need_da) {
-      // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}
-      st.yr[(j - 1) & (NP - 1)] = Num<S>::conj(yval_at(st.yb, st.zib, a.n, st.T, t - 1 - NP));
+      // slide the window: slot (t-1) mod NP now receives y_{t-1-NP} (read at an
+      // always-valid address, selected afterwards; unused without da)
+      const int64_t u = t - 1 - NP;
+      const S yv = *yw_addr(st.yb, st.zib, a.n, st.T, u);
+      st.yr[(j - 1) & (NP - 1)] = yw_ok(st.zib, a.n, st.T, u) ? Num<S>::conj(yv) : Num<S>::zero();
     }
     bwd_steps<S, NP, JJ + 1>(a, st, tb);
   }
@@ -984,6 +1066,7 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
   st.zib = a.zi ? a.zi + sp.zo : nullptr;
   st.dxb = a.dx ? a.dx + sp.xo : nullptr;
   st.dAb = a.dA ? a.dA + sp.dao : nullptr;
+  st.sink = reinterpret_cast<S*>(a.carry + cid * NP);
   st.at = sp.at;
   st.T = sp.T;
   st.need_da = a.dA != nullptr || a.dapart != nullptr;
