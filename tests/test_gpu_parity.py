@@ -108,11 +108,11 @@ def test_backward_serial_strategy(dtype):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("N", [2, 16, 32])
-@pytest.mark.parametrize("T", [3000, 2998, 17])
+@pytest.mark.parametrize("T", [3000, 2998, 17, 8])
 def test_time_invariant(dtype, N, T):
     """a[B, N] and its time-broadcast view. T = 3000 takes the float4 streams of
     the float32 kernels, T = 2998 (not a multiple of 4) the per-sample ones, and
-    T = 17 is a single chunk."""
+    T = 17 and T = 8 are single chunks (8 with float4 streams)."""
     rng = np.random.default_rng(N + T)
     B = 4
     # k ~ U(-0.8, 0.8) puts poles within 1e-8 of the unit circle at N = 32, where
@@ -312,7 +312,7 @@ def _offset_view(a):
 
 
 @pytest.mark.parametrize("N", [2, 4, 8])
-@pytest.mark.parametrize("T,chunk", [(64, 16), (100, 32), (1000, 0), (4096, 48), (2048, 0)])
+@pytest.mark.parametrize("T,chunk", [(4, 0), (8, 16), (64, 16), (100, 32), (1000, 0), (4096, 48), (2048, 0)])
 def test_float4_stream_reruns(N, T, chunk):
     """Float32 time-varying orders 2, 4 and 8 with T a multiple of 4 take the
     float4-stream reruns (chunk 0 reads zi through its y window); the same data
