@@ -260,6 +260,67 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   }
 }
 
+// Phase 1 (adjoint): summaries of chunks c = 1 .. P-1, NP lanes per chunk, in
+// forward time. Lane r runs the forward recurrence with zero input from a unit
+// at position t0-1-r (the state entering the chunk): its outputs h_r(t) are
+// d y_t / d y_{t0-1-r}. The adjoint chunk map is the conjugate transpose of the
+// forward one, so its matrix row r is Psi[r][k] = conj(h_r(t1-1-k)) (the lane's
+// final window), and its zero-state response to g is w[r] = sum_t g_t conj(h_r(t)).
+// One pass gives both. The adjoint-form kernel above needs an extra lane per
+// chunk for w, and at order 32 its 33-lane groups straddle the waves (measured
+// in the ragged batch: 1248 -> 606 us); at the lower orders it is the faster
+// of the two, so this form serves order 32 only. Positions past the
+// end of the signal have zero rows and g = 0 (staged as zeros), so they add
+// nothing to w; Psi of a signal's last chunk meets the zero state.
+template <typename S, int NP>
+__global__ __launch_bounds__(256) void bwd_summary_ff(ChunkArgs<S> a) {
+  using C = ChunkCfg<S, NP, NP>;
+  using W = typename Wide<S>::type;
+  __shared__ W sA[C::G * C::ASTRIDE];
+  __shared__ W sX[C::G * C::TB];
+  __shared__ ItemTab<S, NP, NP> tab;
+  const int64_t first = (int64_t)blockIdx.x * C::G;
+  fill_items<S, NP, NP>(a, tab, first, false);
+  __syncthreads();
+  const int g = threadIdx.x / NP;
+  const int r = threadIdx.x - g * NP;
+  const bool active = g < C::G && tab.rem[g] > 0;
+
+  W ring[NP];  // h_r(u) in slot u mod NP
+#pragma unroll
+  for (int k = 0; k < NP; ++k) ring[k] = (k == NP - 1 - r) ? Num<W>::one() : Num<W>::zero();
+  W w = Num<W>::zero();
+
+  for (int t0 = 0; t0 < a.L; t0 += C::TB) {
+    stage_block<S, NP, NP>(a, sA, sX, a.x, tab, t0);
+    __syncthreads();
+    if (active) {
+      const W* rowbase = sA + g * C::ASTRIDE;
+#pragma unroll
+      for (int j = 0; j < C::TB; ++j) {
+        const W* row = rowbase + j * NP;
+        // four partial sums keep the step's dependency chain at NP/4 products
+        W acc0 = Num<W>::zero(), acc1 = Num<W>::zero(), acc2 = Num<W>::zero(), acc3 = Num<W>::zero();
+#pragma unroll
+        for (int i = 1; i <= NP; ++i) {
+          W& acc = (i & 3) == 1 ? acc0 : (i & 3) == 2 ? acc1 : (i & 3) == 3 ? acc2 : acc3;
+          acc = Num<W>::fms(acc, row[i - 1], ring[(j - i) & (NP - 1)]);
+        }
+        const W y = (acc0 + acc1) + (acc2 + acc3);
+        ring[j & (NP - 1)] = y;
+        w = Num<W>::fma(w, sX[g * C::TB + j], Num<W>::conj(y));
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    W* out = a.summ + (first + g) * C::SUMM;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) out[r * NP + k] = Num<W>::conj(ring[NP - 1 - k]);
+    out[NP * NP + r] = w;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Phase 2: carries, per signal.
 // Forward: carry[c] = state entering chunk c, carry[0] = zi, carry[c+1] = Phi_c carry[c] + z_c.
@@ -1679,8 +1740,14 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   if (a.P > 1) {
-    const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
-    hipLaunchKernelGGL((bwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
+    if constexpr (NP == 32) {
+      using CB = ChunkCfg<S, NP, NP>;
+      const int64_t grid = (summary_items(a) + CB::G - 1) / CB::G;
+      hipLaunchKernelGGL((bwd_summary_ff<S, NP>), dim3((unsigned)grid), dim3(CB::THREADS), 0, st, a);
+    } else {
+      const int64_t grid = (summary_items(a) + C::G - 1) / C::G;
+      hipLaunchKernelGGL((bwd_summary<S, NP>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, a);
+    }
   }
   bool blocked = false;
   if constexpr (blocked_fits<S, NP>()) {
