@@ -610,9 +610,26 @@ __device__ __forceinline__ void load_row_u(S (&row)[NP], const ChunkArgs<S>& a, 
 #pragma unroll
   for (int i = 0; i < NP; ++i) row[i] = keep_bits(coef(a, p[i < a.n ? i : a.n - 1]), i < a.n);
 }
-// The plain form, for steps that sit under a branch of their own.
+// The plain form, for steps that sit under a branch of their own. Float32 rows
+// of a multiple-of-4 order that start on a 16-byte boundary (always in a
+// uniform batch with an aligned A; per signal in a ragged one) load as float4:
+// a lane's 4-byte loads are L2 requests of their own.
 template <typename S, int NP>
 __device__ __forceinline__ void load_row(S (&row)[NP], const ChunkArgs<S>& a, const S* p) {
+  if constexpr (std::is_same<S, float>::value && NP % 4 == 0) {
+    if (a.n == NP && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+      const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+      for (int k = 0; k < NP / 4; ++k) {
+        const float4 v = q[k];
+        row[4 * k] = coef(a, v.x);
+        row[4 * k + 1] = coef(a, v.y);
+        row[4 * k + 2] = coef(a, v.z);
+        row[4 * k + 3] = coef(a, v.w);
+      }
+      return;
+    }
+  }
   if (a.n == NP) {
 #pragma unroll
     for (int i = 0; i < NP; ++i) row[i] = coef(a, p[i]);
@@ -1043,9 +1060,23 @@ __device__ __forceinline__ void bwd_step_branchy(const ChunkArgs<S>& a, BwdLane<
     if (st.need_da) {
       if (st.dAb) {
         S* drow = st.dAb + t * a.n;
+        S d[NP];
 #pragma unroll
-        for (int i = 1; i <= NP; ++i)
-          if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
+        for (int i = 1; i <= NP; ++i) d[i - 1] = coef(a, convert<S>(-(lam * convert<W>(st.yr[(j - i) & (NP - 1)]))));
+        bool vec = false;
+        if constexpr (std::is_same<S, float>::value && NP % 4 == 0) {
+          if (a.n == NP && (reinterpret_cast<uintptr_t>(drow) & 15) == 0) {
+            float4* q = reinterpret_cast<float4*>(drow);
+#pragma unroll
+            for (int k = 0; k < NP / 4; ++k) q[k] = make_float4(d[4 * k], d[4 * k + 1], d[4 * k + 2], d[4 * k + 3]);
+            vec = true;
+          }
+        }
+        if (!vec) {
+#pragma unroll
+          for (int i = 1; i <= NP; ++i)
+            if (i <= a.n) drow[i - 1] = d[i - 1];
+        }
       } else {
 #pragma unroll
         for (int i = 1; i <= NP; ++i)
