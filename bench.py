@@ -207,8 +207,10 @@ def workload_inputs(name, device, seed):
     raise ValueError(name)
 
 
-def bench_workload(name, steps, warmup):
-    """fwd(+bwd) of a secondary workload; returns the JSON record."""
+def bench_workload(name, steps, warmup, ragged_order=0):
+    """fwd(+bwd) of a secondary workload; returns the JSON record. For the ragged
+    workload, ragged_order > 0 runs only that order's group of signals (on the
+    caller's stream), for a kernel trace of one group."""
     import torch
 
     import torchlpc_amd
@@ -245,14 +247,23 @@ def bench_workload(name, steps, warmup):
             ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True)
 
         def step():
+            if ragged_order:
+                for grp in plan.groups:
+                    if grp[0] == ragged_order:
+                        fwd(grp)
+                        bwd(grp)
+                return
             # the order groups run on the plan's side streams (graph branches)
             plan.run(fwd)
             plan.run(bwd)
 
-        S = sum(lengths)
-        alg = sum((3 * N + 5) * 4 * T for T, N in zip(lengths, orders))
+        sel = [(T, N) for T, N in zip(lengths, orders) if not ragged_order or N == ragged_order]
+        S = sum(T for T, _ in sel)
+        alg = sum((3 * N + 5) * 4 * T for T, N in sel)
         workload = f"ragged all-pole filter fwd + fused adjoint bwd, {K} signals, T_k~U{{1024..65536}}, N_k in {{2,4,8,16,32}}, fp32, one HIP graph"
-        cfg = {"workload": workload, "signals": K, "samples": S}
+        if ragged_order:
+            workload += f", only the {len(sel)} signals of order {ragged_order}"
+        cfg = {"workload": workload, "signals": len(sel), "samples": S}
         s = torch.cuda.Stream()
         s.wait_stream(stream)
         with torch.cuda.stream(s):
@@ -403,6 +414,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--ragged-order", type=int, default=0,
+                    help="with --workload cfg4: run only the signals of this order (one group)")
     ap.add_argument("--workload", default="headline", choices=["headline", "cfg1", "cfg3", "cfg4", "cfg5"],
                     help="secondary workloads print their own line (the default is the headline)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -429,7 +442,7 @@ def main():
 
     if args.workload != "headline":
         log(f"[bench] workload {args.workload}")
-        print(json.dumps(bench_workload(args.workload, args.steps, args.warmup)), flush=True)
+        print(json.dumps(bench_workload(args.workload, args.steps, args.warmup, args.ragged_order)), flush=True)
         return
 
     import torch
