@@ -1,6 +1,8 @@
 // C ABI of libtorchlpc_amd_hip.so (declared in include/torchlpc_amd.h).
 // Validates sizes, carves the caller's workspace, picks a strategy and launches.
 #include <stddef.h>
+
+#include <atomic>
 #include <stdint.h>
 
 #include "../../include/torchlpc_amd.h"
@@ -61,15 +63,17 @@ inline int pick_chunk(int64_t B, int64_t T, int np, int requested) {
 
 // Compute units of the current device (cached per device).
 inline int device_cus() {
-  static int cus[64] = {0};
+  // one slot per device, filled on first use; concurrent callers store the same
+  // value, so relaxed atomics suffice
+  static std::atomic<int> cus[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int v = 0;
+  int v = cus[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-    cus[dev] = v;
+    cus[dev].store(v, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return v;
 }
 
 // Order-16 float32 chunks (the quad kernels): at most 64 chunks per compute
