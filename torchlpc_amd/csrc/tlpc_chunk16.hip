@@ -621,40 +621,47 @@ __global__ __launch_bounds__(CARRY_THREADS) void carry16(ChunkArgs<float> a) {
   const double* summ = a.summ + b * links * SUMM;
   const double* wsum = FWD ? nullptr : a.wsum + b * links * NP;
   double* carry = a.carry + b * P * NP;
-  constexpr int PER = CB * SUMM / 64;  // doubles per lane per batch
-  double stage[PER];
-  // Loads are unconditional, at a link index clamped into the chain (the chain
-  // never reads the slots past its end), with the adjoint's choices made on the
-  // address; a load under a branch would be waited for at once.
+  constexpr int PL = (SUMM + 63) / 64;  // doubles per loader lane per link
+  double stage[CB][PL];
+  // A loader lane takes elements lane, lane+64, ... of each link of its batch:
+  // one base per link plus a constant, no division by the link size. Loads are
+  // unconditional, at a link index clamped into the chain and an element index
+  // clamped into the link (the chain never reads those slots), with the
+  // adjoint's choices made on the address; a load under a branch would be
+  // waited for at once.
   auto fetch = [&](int64_t j) {
     const int64_t l0 = j * CB;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = q * 64 + lane;
-      const int li = e / SUMM, off = e - li * SUMM;
+    for (int li = 0; li < CB; ++li) {
       const int64_t l = min(l0 + li, links - 1);
-      double v;
       if constexpr (FWD) {
-        v = summ[l * SUMM + off];
+        const double* src = summ + l * SUMM;
+#pragma unroll
+        for (int q = 0; q < PL; ++q) stage[li][q] = src[min(q * 64 + lane, SUMM - 1)];
       } else {
-        // link l is chunk P-1-l: its w, and Phi^T of the forward summary of that
-        // chunk (link 0, the last chunk, meets the zero state and has no matrix)
-        const bool w = off >= NP * NP;
-        const double* p = w ? wsum + (links - 1 - l) * NP + (off - NP * NP)
-                            : summ + (links - (l > 0 ? l : 1)) * SUMM + off;
-        const double raw = *p;
-        v = (w || l > 0) ? raw : 0.0;
+        // link l is chunk P-1-l: Phi^T of the forward summary of that chunk (link
+        // 0, the last chunk, meets the zero state and has no matrix), then its w
+        const double* ms = summ + (links - (l > 0 ? l : 1)) * SUMM;
+        const double* ws = wsum + (links - 1 - l) * NP;
+#pragma unroll
+        for (int q = 0; q < PL; ++q) {
+          const int off = q * 64 + lane;
+          const bool w = off >= NP * NP;
+          const double raw = w ? ws[min(off - NP * NP, NP - 1)] : ms[off];
+          stage[li][q] = (w || l > 0) ? raw : 0.0;
+        }
       }
-      stage[q] = v;
     }
   };
   auto commit = [&](int64_t j) {
     double(*dst)[LINKP] = buf[j & 1];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = q * 64 + lane;
-      const int li = e / SUMM, off = e - li * SUMM;
-      dst[li][off < NP * NP ? off + off / NP : off + NP] = stage[q];
+    for (int li = 0; li < CB; ++li) {
+#pragma unroll
+      for (int q = 0; q < PL; ++q) {
+        const int off = q * 64 + lane;
+        if (off < SUMM) dst[li][off < NP * NP ? off + off / NP : off + NP] = stage[li][q];
+      }
     }
   };
   double s = 0.0;
