@@ -173,7 +173,7 @@ def workload_inputs(name, device, seed):
 
     g = torch.Generator(device=device).manual_seed(seed)
     f64 = dict(device=device, dtype=torch.float64)
-    if name == "cfg1":  # B=16, T=8192, N=4: two resonators per signal, 32 anchor frames
+    if name == "b16-t8192-n4":  # B=16, T=8192, N=4: two resonators per signal, 32 anchor frames
         B, T = 16, 8192
         r = torch.rand(B, 32, 2, generator=g, **f64) * 0.45 + 0.5
         th = (torch.rand(B, 32, 2, generator=g, **f64) * 0.9 + 0.05) * math.pi
@@ -189,7 +189,7 @@ def workload_inputs(name, device, seed):
             -1,
         )
         return torch.randn(B, T, generator=g, **f64), A, torch.zeros(B, 4, **f64), None
-    if name == "cfg3":  # B=4, T=2^20, N=2: slowly drifting resonators near the unit circle
+    if name == "b4-t1048576-n2":  # B=4, T=2^20, N=2: slowly drifting resonators near the unit circle
         B, T = 4, 1 << 20
         r = torch.rand(B, 1, generator=g, **f64) * 0.099 + 0.9
         thc = (torch.rand(B, 1, generator=g, **f64) * 0.6 + 0.2) * math.pi
@@ -199,7 +199,7 @@ def workload_inputs(name, device, seed):
         A = torch.stack([-2 * r * torch.cos(th), (r * r).expand(B, T)], -1)
         return (torch.randn(B, T, generator=g, **f64), A, torch.randn(B, 2, generator=g, **f64),
                 torch.randn(B, T, generator=g, **f64))
-    if name == "cfg5":  # B=1024, T=16384, N=32, time-invariant a[B, N]
+    if name == "ti-b1024-t16384-n32":  # B=1024, T=16384, N=32, time-invariant a[B, N]
         B, T, N = 1024, 16384, 32
         a = _step_up((torch.rand(B, N, generator=g, **f64) * 2 - 1) * 0.8)
         return (torch.randn(B, T, generator=g, **f64), a, torch.randn(B, N, generator=g, **f64) * 0.1,
@@ -219,7 +219,7 @@ def bench_workload(name, steps, warmup, ragged_order=0):
     ops = torch.ops.torchlpc_amd
     stream = torch.cuda.current_stream()
     f32 = torch.float32
-    if name == "cfg4":  # 512 ragged signals, lengths ~ U{1024..65536}, orders in {2,4,8,16,32}
+    if name == "ragged512":  # 512 ragged signals, lengths ~ U{1024..65536}, orders in {2,4,8,16,32}
         g = torch.Generator(device="cpu").manual_seed(1004)
         K = 512
         lengths = torch.randint(1024, 65537, (K,), generator=g).tolist()
@@ -274,13 +274,13 @@ def bench_workload(name, steps, warmup, ragged_order=0):
             step()
         run = graph.replay
     else:
-        x64, A64, zi64, w64 = workload_inputs(name, dev, {"cfg1": 1001, "cfg3": 1003, "cfg5": 1005}[name])
+        x64, A64, zi64, w64 = workload_inputs(name, dev, {"b16-t8192-n4": 1001, "b4-t1048576-n2": 1003, "ti-b1024-t16384-n32": 1005}[name])
         x, A, zi = x64.to(f32), A64.to(f32).contiguous(), zi64.to(f32)
         w = w64.to(f32) if w64 is not None else None
         B, T = x.shape
         N = A.shape[-1]
         S = B * T
-        if name == "cfg1":
+        if name == "b16-t8192-n4":
             def run():
                 ops.lpc(x, A, zi)
             alg = (N + 2) * 4 * S
@@ -415,9 +415,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--ragged-order", type=int, default=0,
-                    help="with --workload cfg4: run only the signals of this order (one group)")
-    ap.add_argument("--workload", default="headline", choices=["headline", "cfg1", "cfg3", "cfg4", "cfg5"],
-                    help="secondary workloads print their own line (the default is the headline)")
+                    help="with --workload ragged512: run only the signals of this order (one group)")
+    ap.add_argument("--workload", default="b256-t48000-n16", choices=["b256-t48000-n16", "b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32"],
+                    help="the other workloads print their own line (the default is B=256, T=48000, N=16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one copy-like kernel of known size (for counter scaling)")
     ap.add_argument("--trace-summary", metavar="DB", help="summarise a rocprofv3 kernel-trace database and exit")
@@ -434,13 +434,13 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(
             "bench.py measures one GPU: this build runs the filter on a single MI355X "
-            "(signals shard across GPUs with no communication; see DESIGN.md). "
+            "(signals shard across GPUs with no communication). "
             f"--gpus {args.gpus} is not supported.",
             file=sys.stderr,
         )
         sys.exit(2)
 
-    if args.workload != "headline":
+    if args.workload != "b256-t48000-n16":
         log(f"[bench] workload {args.workload}")
         print(json.dumps(bench_workload(args.workload, args.steps, args.warmup, args.ragged_order)), flush=True)
         return

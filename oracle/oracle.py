@@ -179,7 +179,8 @@ def band_matrix(A_b: np.ndarray) -> np.ndarray:
 
 # --------------------------------------------------------------------------
 # Seeded workload generators (FP64 on the CPU, then cast), one per benchmark
-# configuration. All use numpy's PCG64 with seed 1000 + config number.
+# workload, named for what it is and its sizes. All use numpy's PCG64 with
+# seeds 1001..1005.
 
 
 def step_up(k: np.ndarray) -> np.ndarray:
@@ -209,7 +210,7 @@ def _interp_anchors(values: np.ndarray, T: int, hop: int) -> np.ndarray:
     return values[:, lo] * (1 - frac) + values[:, hi] * frac
 
 
-def make_config1(B=16, T=8192, N=4, seed=1001):
+def make_resonators_b16_t8192_n4(B=16, T=8192, N=4, seed=1001):
     """Plain forward: 2 conjugate pole pairs per signal, (r, theta) at 32 anchors."""
     assert N == 4
     rng = np.random.default_rng(seed)
@@ -247,11 +248,11 @@ def make_reflection_inputs(B, T, N, hop, seed, kmax=0.9, zi_std=0.1):
     return x, A, zi, w
 
 
-def make_config2(B=256, T=48000, N=16, seed=1002):
+def make_reflection_b256_t48000_n16(B=256, T=48000, N=16, seed=1002):
     return make_reflection_inputs(B, T, N, 240, seed)
 
 
-def make_config3(B=4, T=1 << 20, seed=1003):
+def make_drifting_b4_t1m_n2(B=4, T=1 << 20, seed=1003):
     """Time-varying resonators: r ~ U(0.9, 0.999) per signal, angle drifting as a
     slow sinusoid (period 2^16): theta_t = theta_c + 0.1 pi sin(2 pi t / 2^16 + phi)."""
     rng = np.random.default_rng(seed)
@@ -267,10 +268,10 @@ def make_config3(B=4, T=1 << 20, seed=1003):
     return x, A, zi, w
 
 
-def make_config4(count=512, seed=1004, lmin=1024, lmax=65536, orders=(2, 4, 8, 16, 32)):
+def make_ragged_512(count=512, seed=1004, lmin=1024, lmax=65536, orders=(2, 4, 8, 16, 32)):
     """Ragged batch: lengths ~ U{lmin..lmax}, orders ~ U(orders); per-signal arrays.
 
-    Reflection coefficients k ~ U(-0.9, 0.9) (the headline recipe) for orders up
+    Reflection coefficients k ~ U(-0.9, 0.9) (as for B=256, T=48000) for orders up
     to 16 and U(-0.5, 0.5) at order 32: with 0.9 an order-32 time-varying filter
     grows to 1e28..1e116 over these lengths, past the float32 range of any
     implementation."""
@@ -291,7 +292,7 @@ def make_config4(count=512, seed=1004, lmin=1024, lmax=65536, orders=(2, 4, 8, 1
     return sigs
 
 
-def make_config5(B=1024, T=16384, N=32, seed=1005):
+def make_ti_b1024_t16384_n32(B=1024, T=16384, N=32, seed=1005):
     """Time-invariant: one stable coefficient row per signal from k ~ U(-0.8, 0.8)."""
     rng = np.random.default_rng(seed)
     x = rng.standard_normal((B, T))

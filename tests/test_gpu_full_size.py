@@ -1,11 +1,11 @@
-"""Workload configurations at their full sizes, and HIP-graph capture.
+"""Benchmark workloads at their full sizes, and HIP-graph capture.
 
-* config 1: B=16, T=8192, N=4, float32 forward (+ zf), time-varying resonators;
-* config 3: B=4, T=2^20, N=2, float32 and float64 forward + backward;
-* config 5: B=1024, T=16384, N=32, float32, time-invariant a[B, N] and its da.
+* B=16, T=8192, N=4, float32 forward (+ zf), time-varying resonators;
+* B=4, T=2^20, N=2, float32 and float64 forward + backward, drifting resonators;
+* B=1024, T=16384, N=32, float32, time-invariant a[B, N] and its da.
 
 Each output is checked against the FP64 oracle with the stated bars
-(float32 <= 1e-5, float64 <= 1e-12, relative L2). Config 5's inputs are
+(float32 <= 1e-5, float64 <= 1e-12, relative L2). The time-invariant inputs are
 ill-conditioned (k ~ U(-0.8, 0.8) at N = 32 puts poles close to the unit
 circle): the reference's own float32 loop is at 5e-4 there. The kernels
 carry the recurrence in float64, which keeps float32 outputs inside the bar.
@@ -28,10 +28,10 @@ def ops():
     return torch.ops.torchlpc_amd
 
 
-def test_config1_forward_and_zf():
+def test_resonators_b16_t8192_n4_forward_and_zf():
     import torchlpc_amd
 
-    x, A, zi = (v.astype(np.float32) for v in O.make_config1())
+    x, A, zi = (v.astype(np.float32) for v in O.make_resonators_b16_t8192_n4())
     y, zf = torchlpc_amd.sample_wise_lpc(dev(x), dev(A), dev(zi), return_zf=True)
     y64 = O.lpc(x.astype(np.float64), A.astype(np.float64), zi.astype(np.float64))
     assert rel(host(y), y64) <= 1e-5
@@ -39,8 +39,8 @@ def test_config1_forward_and_zf():
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-12)])
-def test_config3_full_size(dtype, tol):
-    x, A, zi, w = (v.astype(dtype) for v in O.make_config3())
+def test_drifting_b4_t1m_n2_full_size(dtype, tol):
+    x, A, zi, w = (v.astype(dtype) for v in O.make_drifting_b4_t1m_n2())
     up = lambda v: v.astype(np.float64)  # noqa: E731
     yd = ops().lpc(dev(x), dev(A), dev(zi))
     dx, dA, dzi = ops().lpc_backward(dev(w), dev(A), dev(zi), yd, True, True, True)
@@ -50,8 +50,8 @@ def test_config3_full_size(dtype, tol):
     assert max(errs) <= tol, errs
 
 
-def test_config5_full_size_time_invariant():
-    x, a, zi, w = (v.astype(np.float32) for v in O.make_config5())
+def test_time_invariant_b1024_t16384_n32_full_size():
+    x, a, zi, w = (v.astype(np.float32) for v in O.make_ti_b1024_t16384_n32())
     B, T = x.shape
     N = a.shape[1]
     yd = ops().lpc(dev(x), dev(a), dev(zi))

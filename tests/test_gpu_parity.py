@@ -207,11 +207,11 @@ def test_zi_zf_chaining():
 
 
 def test_reference_cpu_op_fp32_agreement():
-    """GPU fp32 vs the reference's own compiled CPU op (fp32) on config-1-shaped inputs."""
+    """GPU fp32 vs the reference's own compiled CPU op (fp32) on the B=16, T=8192, N=4 resonators."""
     if not O.REF_LIB_PATH.exists():
         pytest.skip("reference CPU op not built")
     torch.ops.load_library(str(O.REF_LIB_PATH))
-    x, A, zi = (v.astype(np.float32) for v in O.make_config1())
+    x, A, zi = (v.astype(np.float32) for v in O.make_resonators_b16_t8192_n4())
     ref32 = torch.ops.torchlpc.lpc(torch.from_numpy(x), torch.from_numpy(A), torch.from_numpy(zi)).numpy()
     y = host(ops().lpc(dev(x), dev(A), dev(zi)))
     y64 = O.lpc(x.astype(np.float64), A.astype(np.float64), zi.astype(np.float64))
@@ -220,7 +220,7 @@ def test_reference_cpu_op_fp32_agreement():
 
 
 @pytest.mark.parametrize("B,T", [(8, 48000), (2, 2000)])
-def test_headline_inputs_fp32(B, T):
+def test_reflection_n16_inputs_fp32(B, T):
     """Reflection-coefficient inputs of the B=256, T=48000, N=16 benchmark (k ~ U(-0.9, 0.9)
     every 240 samples), on a subset of signals: y, dx, dA, dzi within 1e-5 of FP64."""
     x, A, zi, w = O.make_reflection_inputs(B, T, 16, 240, 1002)

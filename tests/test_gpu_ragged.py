@@ -84,7 +84,7 @@ def test_ragged_full_workload_and_graph_replay():
     from torchlpc_amd import RaggedPlan
     from torchlpc_amd.ragged import RaggedLPC
 
-    sigs = O.make_config4()
+    sigs = O.make_ragged_512()
     lengths = [len(s[0]) for s in sigs]
     orders = [s[1].shape[1] for s in sigs]
     x, a, zi, g = _pack(sigs, np.float32)
