@@ -4,8 +4,10 @@
 //   y[t] = x[t] - sum_{i=1..N} A[t,i] y[t-i]
 // The state s_t = [y_{t-1}, ..., y_{t-N}] evolves by a companion matrix, so a chunk
 // of L samples is the affine map s_out = Phi s_in + z. Three phases:
-//   1. summary: per chunk, Phi (N homogeneous basis responses) and z (zero-state
-//      response), computed by N+1 lanes that share the chunk's A rows from LDS;
+//   1. summary: per chunk, Phi and z, computed in the wide type by NP (forward:
+//      an adjoint sweep per row of Phi) or NP+1 (adjoint) lanes that share the
+//      chunk's A rows from LDS; at order 32 the adjoint summary runs in forward
+//      form with NP lanes;
 //   2. carry:   per signal, s_{c+1} = Phi_c s_c + z_c from s_0 = zi;
 //   3. rerun:   per chunk, one lane re-filters its L samples from the carry and
 //               writes y.
