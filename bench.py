@@ -499,6 +499,7 @@ def main():
     ms = total_ms / n
     fwd_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in ev)
     bwd_ms = statistics.mean(e[1].elapsed_time(e[2]) for e in ev)
+    step_median_ms = statistics.median(e[0].elapsed_time(e[2]) for e in ev)
 
     S = B * T
     bytes_fwd = (N + 2) * 4 * S
@@ -548,6 +549,23 @@ def main():
             "kernels": list(OP_KERNELS.get(dom, ())),
         },
     }
+
+    # cold steps: each one after a 1 GiB write that evicts the L2s and the MALL,
+    # timed alone (the median of 10); the working set of a step (>2.6 GB) is far
+    # beyond the MALL anyway, so warm and cold agree closely
+    flush = torch.empty(COPY_BYTES // 4, device=dev)
+    cold = []
+    for _ in range(10):
+        flush.fill_(1.0)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        step()
+        c1.record(stream)
+        torch.cuda.synchronize()
+        cold.append(c0.elapsed_time(c1))
+    del flush
+    result["step_median_ms"] = step_median_ms
+    result["cold_step_median_ms"] = statistics.median(cold)
 
     # a device copy of COPY_BYTES, timed the same way: the bandwidth a plain
     # streaming kernel reaches on this GPU, reported beside the published peak
