@@ -132,7 +132,8 @@ struct Plan {
   int64_t P = 0;
   bool c16 = false;  // float32 order-16 time-varying kernels
   size_t summ_bytes = 0;
-  size_t summ_off = 0, carry_off = 0, wsum_off = 0, dapart_off = 0, total = 0;
+  size_t summ_off = 0, carry_off = 0, wsum_off = 0, dapart_off = 0, ypad_off = 0, total = 0;
+  bool ypad = false;  // time-invariant adjoint with da partials: the y window's zi blocks
 };
 
 // The order-16 float32 kernels serve time-varying filters with chunks that are
@@ -184,7 +185,9 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   p.carry_off = p.summ_bytes;
   p.wsum_off = p.carry_off + align_up(carry);
   p.dapart_off = p.wsum_off + align_up(wsum);
-  p.total = p.dapart_off + align_up(dapart);
+  p.ypad = dapart > 0;
+  p.ypad_off = p.dapart_off + align_up(dapart);
+  p.total = p.ypad_off + (p.ypad ? align_up((size_t)B * ti_ypad(p.np) * es) : 0);
   return p;
 }
 
@@ -276,6 +279,7 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   a.wsum = reinterpret_cast<W*>(w + p.wsum_off);
   if (need_da && da_reduced) {
     a.dapart = reinterpret_cast<W*>(w + p.dapart_off);
+    if (p.ypad) a.ypad = reinterpret_cast<S*>(w + p.ypad_off);
     a.da_ti = static_cast<S*>(dA);
   } else if (need_da) {
     a.dA = static_cast<S*>(dA);

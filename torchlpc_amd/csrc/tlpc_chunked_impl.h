@@ -690,6 +690,8 @@ __device__ __forceinline__ void v4set(float4& v, float f) {
 __device__ __forceinline__ float4 ld4(const float* p, int64_t i) {
   return *reinterpret_cast<const float4*>(p + i);
 }
+constexpr int floordiv4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+constexpr int posmod(int v, int m) { return ((v % m) + m) % m; }
 
 // Keeps a step's loads where the source puts them (see tlpc_chunk16.hip).
 __device__ __forceinline__ void pin_step() { __builtin_amdgcn_sched_barrier(0); }
@@ -834,13 +836,27 @@ struct BwdLaneTI {
   S gs[VEC ? 1 : RC::DS];           // g_t in slot (tstart - t) mod DS
   float4 gq[VEC ? RC::DS / 4 : 1];  // VEC: g[u .. u+3] (u a multiple of 4), by step block
   float4 dq;                        // VEC: lambda of the current four steps
-  S ys[RC::DS];  // y_{t-1-NP} (window input of step t), same slots
+  S ys[VEC ? 1 : RC::DS];           // y_{t-1-NP} (window input of step t), same slots
+  float4 yq[VEC ? RC::DS / 4 : 1];  // VEC: y blocks of the window input (positions t-1-NP)
+  const float* ypadb;               // VEC: this signal's y values below position 0
   const S* gb;
   const S* yb;
   const S* zib;
   S* dxb;
   int64_t T;
 };
+
+// y[u .. u+3] (u a multiple of 4) for the VEC adjoint's window: below position 0
+// from the signal's ypad blocks (zi), above from y clamped into the signal
+// (values past T meet only steps past the end, whose lambda is zero). The choice
+// is made on the address, so the load is never under a branch; blocks below the
+// ypad range are never read by a step (they are the prefetch past the walk).
+template <typename S, int NP, bool VEC>
+__device__ __forceinline__ float4 ywin_ti(const BwdLaneTI<S, NP, VEC>& st, int64_t u) {
+  const float* p = u < 0 ? st.ypadb + (u + ti_ypad(NP) > 0 ? u + ti_ypad(NP) : 0)
+                         : st.yb + (u < st.T - 4 ? u : st.T - 4);
+  return *reinterpret_cast<const float4*>(p);
+}
 
 // Steps t = tb - JJ for JJ = 0..K-1 (t mod NP = NP-1-JJ mod NP), unrolled by
 // template recursion so that every ring index is a compile-time constant (a
@@ -883,9 +899,20 @@ __device__ __forceinline__ void bwd_steps_ti(const ChunkArgs<S>& a, BwdLaneTI<S,
         st.dacc[i - 1] = Num<W>::fms(st.dacc[i - 1], lam, convert<W>(st.yr[(j - i) & (NP - 1)]));
       // slide the window: slot (t-1) mod NP now receives y_{t-1-NP}; its stream
       // slot is consumed here, before it is reloaded
-      const bool ynew_ok = yw_ok(st.zib, a.n, st.T, t - 1 - NP);
-      st.yr[(j - 1) & (NP - 1)] = ynew_ok ? Num<S>::conj(st.ys[JJ % RC::DS]) : Num<S>::zero();
-      st.ys[JJ % RC::DS] = *yw_addr(st.yb, st.zib, a.n, st.T, t - RC::DS - 1 - NP);
+      if constexpr (VEC) {
+        // y_{t-1-NP} sits at element (2 - JJ - NP) mod 4 of block slot
+        // (QB - 1 + floor((2 - JJ - NP) / 4)) mod QB: tb = 4m + 3 with m = QB - 1
+        // mod QB at every unrolled block (chunk ends are multiples of 16)
+        constexpr int QB = RC::DS / 4;
+        constexpr int ye = posmod(2 - JJ - NP, 4);
+        constexpr int yslot = posmod(QB - 1 + floordiv4(2 - JJ - NP), QB);
+        st.yr[(j - 1) & (NP - 1)] = v4get<ye>(st.yq[yslot]);
+        if constexpr (ye == 0) st.yq[yslot] = ywin_ti(st, t - 1 - NP - RC::DS);
+      } else {
+        const bool ynew_ok = yw_ok(st.zib, a.n, st.T, t - 1 - NP);
+        st.yr[(j - 1) & (NP - 1)] = ynew_ok ? Num<S>::conj(st.ys[JJ % RC::DS]) : Num<S>::zero();
+        st.ys[JJ % RC::DS] = *yw_addr(st.yb, st.zib, a.n, st.T, t - RC::DS - 1 - NP);
+      }
     }
     pin_step();
     bwd_steps_ti<S, NP, DA, VEC, JJ + 1>(a, st, tb);
@@ -915,7 +942,17 @@ __global__ __launch_bounds__(256) void bwd_rerun_ti(ChunkArgs<S> a) {
   for (int k = 0; k < RC::DS; ++k) {
     const int64_t t = tstart - k;
     if constexpr (!VEC) st.gs[k] = opaque_v(st.gb[clamp64(t, 0, a.T - 1)]);
-    if constexpr (DA == DA_PARTIAL) st.ys[k] = opaque_v(*yw_addr(st.yb, st.zib, a.n, a.T, t - 1 - NP));
+    if constexpr (DA == DA_PARTIAL && !VEC) st.ys[k] = opaque_v(*yw_addr(st.yb, st.zib, a.n, a.T, t - 1 - NP));
+  }
+  if constexpr (DA == DA_PARTIAL && VEC) {
+    // the y blocks that hold positions tstart-1-NP and the DS - 4 below it (the
+    // slot of block u0 - 4k is a constant once the loop is unrolled)
+    constexpr int QB = RC::DS / 4;
+    st.ypadb = reinterpret_cast<const float*>(a.ypad) + b * ti_ypad(NP);
+    const int64_t u0 = tstart - 1 - NP - posmod(2 - NP, 4);
+#pragma unroll
+    for (int k = 0; k < QB; ++k)
+      st.yq[posmod(QB - 1 + floordiv4(2 - NP) - k, QB)] = opaque_v(ywin_ti(st, u0 - 4 * k));
   }
   if constexpr (VEC) {
 #pragma unroll
@@ -1325,9 +1362,6 @@ __device__ __forceinline__ float4 yblock(const BwdLaneTv4<NP>& st, int64_t u) {
   return make_float4(c[0], c[1], c[2], c[3]);
 }
 
-constexpr int floordiv4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
-constexpr int posmod(int v, int m) { return ((v % m) + m) % m; }
-
 // Steps t = tb - JJ (tb is 3 mod 4 and NP-1 mod NP). Block slot q holds
 // positions [u, u+3] with u = t - e, e = 3 - JJ % 4. The window input
 // y_{t-1-NP} sits at element (2 - JJ - NP) mod 4 of y slot
@@ -1577,6 +1611,12 @@ __global__ __launch_bounds__(64) void carry_ti(ChunkArgs<S> a) {
   const int64_t b = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t P = a.P;
+  if (!FWD && a.ypad && lane < ti_ypad(NP)) {
+    // the adjoint rerun's y window below position 0: element i is position
+    // i - ti_ypad(NP), y_{-1-k} = zi[k] for k < n, zero beyond
+    const int k = ti_ypad(NP) - 1 - lane;
+    a.ypad[b * ti_ypad(NP) + lane] = (a.zi && k < a.n) ? a.zi[b * a.n + k] : Num<S>::zero();
+  }
   if (lane < NP) {
     // lane k: response to y_{-1-k} = 1 over one whole chunk (column k of Phi)
     W co[NP];
@@ -1750,7 +1790,10 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     hipLaunchKernelGGL((carry_ti<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
     const dim3 g((unsigned)((nch + 255) / 256));
-    const bool vec = ti_vec(a, a.x, a.dx);
+    // with da partials the y window moves in float4 blocks too (y aligned, and
+    // the zi blocks in the workspace)
+    const bool vec = ti_vec(a, a.x, a.dx) &&
+                     (!a.dapart || (a.ypad && (reinterpret_cast<uintptr_t>(a.y) & 15) == 0));
     // a full per-sample dA of a time-invariant filter takes the per-step row
     // kernel, which reads the one row at stride 0
     if (a.dapart) {

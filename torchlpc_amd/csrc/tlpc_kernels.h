@@ -41,7 +41,15 @@ struct ChunkArgs {
   // Summaries and carries are indexed by chunk id; P holds the largest P_k.
   const int64_t* rg;
   int64_t G;
+  // Time-invariant adjoint with da partials (float4 streams): per signal, the y
+  // window's values below position 0 (y_{-1-k} = zi[k], zero past the order)
+  // as ti_ypad(NP) floats ending at position -1, 16-byte aligned; or null.
+  S* ypad;
 };
+
+// Floats per signal of ChunkArgs::ypad: the y window of an order-NP adjoint
+// reaches position -1-NP; whole 4-sample blocks.
+constexpr int ti_ypad(int np) { return (np + 1 + 3) / 4 * 4; }
 
 // One signal's view of the batch (uniform or ragged).
 struct Span {
