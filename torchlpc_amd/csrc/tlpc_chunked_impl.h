@@ -728,7 +728,7 @@ template <typename S, int NP, bool VEC>
 struct FwdLaneTI {
   using W = typename Wide<S>::type;
   using RC = RerunCfg<S, NP>;
-  W ring[NP];                        // y_u in slot u mod NP
+  W ring[NP];                        // pending sums: slot u mod NP collects -a_i y_{u-i}
   W cw[NP];                          // coefficients
   S xs[VEC ? 1 : RC::DS];            // x_t in slot t mod DS
   float4 xq[VEC ? RC::DS / 4 : 1];   // VEC: x[4q' .. 4q'+3] in slot q' mod DS/4
@@ -745,13 +745,18 @@ __device__ __forceinline__ void fwd_steps_ti(FwdLaneTI<S, NP, VEC>& st, int64_t 
   if constexpr (JJ < RC::K) {
     constexpr int j = JJ % NP;  // t mod NP: blocks start at multiples of NP
     const int64_t t = tb + JJ;
+    // pending-sum form: y_t = x_t + p(t), then p(t+i) -= a_i y_t; the step's NP
+    // products are independent and only the first feeds the next step (the
+    // direct form chained NP products per step)
     W acc;
     if constexpr (VEC)
-      acc = convert<W>(v4get<JJ & 3>(st.xq[(JJ % RC::DS) / 4]));
+      acc = convert<W>(v4get<JJ & 3>(st.xq[(JJ % RC::DS) / 4])) + st.ring[j];
     else
-      acc = convert<W>(st.xs[JJ % RC::DS]);
+      acc = convert<W>(st.xs[JJ % RC::DS]) + st.ring[j];
+    st.ring[j] = Num<W>::zero();
 #pragma unroll
-    for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, st.cw[i - 1], st.ring[(j - i) & (NP - 1)]);
+    for (int i = 1; i <= NP; ++i)
+      st.ring[(j + i) & (NP - 1)] = Num<W>::fms(st.ring[(j + i) & (NP - 1)], st.cw[i - 1], acc);
     if constexpr (VEC) {
       // the block's last sample was just read: its slot takes the block DS ahead
       if constexpr ((JJ & 3) == 3) st.xq[(JJ % RC::DS) / 4] = ld4(st.xb, min(t - 3 + RC::DS, st.T - 4));
@@ -759,7 +764,6 @@ __device__ __forceinline__ void fwd_steps_ti(FwdLaneTI<S, NP, VEC>& st, int64_t 
       st.xs[JJ % RC::DS] = st.xb[t + RC::DS < st.T ? t + RC::DS : st.T - 1];
     }
     pin_step();
-    st.ring[j] = acc;
     if constexpr (VEC) {
       v4set<JJ & 3>(st.yq, (float)acc);
       // chunk ends are multiples of 4, so a block is wholly inside or past the end
@@ -797,8 +801,18 @@ __global__ __launch_bounds__(256) void fwd_rerun_ti(ChunkArgs<S> a) {
   }
   load_coefs_w<S, NP>(st.cw, a, a.A + b * a.a_bstride);
   const W* cr = a.carry + cid * NP;
+  // the carry is the window y_{t0-1-r} (r < NP); it leaves the pending sums
+  // p(t0+u) = -sum_{i > u} a_i y_{t0+u-i} for the chunk's first NP positions
+  W win[NP];
 #pragma unroll
-  for (int r = 0; r < NP; ++r) st.ring[NP - 1 - r] = opaque_v(cr[r]);
+  for (int r = 0; r < NP; ++r) win[r] = opaque_v(cr[r]);
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    W pu = Num<W>::zero();
+#pragma unroll
+    for (int i = u + 1; i <= NP; ++i) pu = Num<W>::fms(pu, st.cw[i - 1], win[i - u - 1]);
+    st.ring[u] = pu;
+  }
   for (int jb = 0; jb < len; jb += RC::K) fwd_steps_ti<S, NP, VEC, 0>(st, t0 + jb);
 }
 
