@@ -136,7 +136,20 @@ def cpu_baseline(T, N, sample_signals=64, min_seconds=10.0):
         f"(parallel over signals only, so effective parallelism = min(signals, threads) = "
         f"{min(threads, Bs)}; host os.cpu_count()={cores})",
         "seconds_per_run": t,
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _step_up(k):
@@ -245,6 +258,9 @@ def bench_workload(name, steps, warmup, ragged_order=0):
         def bwd(grp):
             n, table, Kg, G, maxp, L = grp
             ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True)
+
+        if ragged_order and ragged_order not in orders:
+            raise SystemExit(f"--ragged-order {ragged_order}: no signal of that order (orders are 2, 4, 8, 16, 32)")
 
         def step():
             if ragged_order:
