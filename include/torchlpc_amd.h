@@ -73,6 +73,8 @@ int tlpc_abi_version(void);
  * Forward filter. Replaces the reference op torchlpc::lpc(x, A, zi)
  * (torchlpc/csrc/scan_cpu.cpp:143-164 CPU, torchlpc/csrc/cuda/lpc.cu:115-175 CUDA,
  * schema at scan_cpu.cpp:169). `chunk_len` 0 lets the library choose.
+ * B == 0 is a no-op. With T == 0 the per-sample buffers (x, a per-sample A, y;
+ * in the backward g, y, dx and a per-sample dA) hold nothing and may be NULL.
  */
 size_t tlpc_lpc_forward_workspace(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,
                                   int32_t strategy, int32_t chunk_len);

@@ -39,6 +39,7 @@ def lib():
     L.tlpc_scan_workspace.argtypes = [ctypes.c_int, i64, i64, i32, i32]
     L.tlpc_lpc_forward.argtypes = [ctypes.c_int, p, p, p, p, i64, i64, i32, i32, i32, i32, p, sz, p]
     L.tlpc_lpc_backward.argtypes = [ctypes.c_int, p, p, p, p, p, p, p, i64, i64, i32, i32, i32, i32, p, sz, p]
+    L.tlpc_scan.argtypes = [ctypes.c_int, p, p, p, p, i64, i64, i32, p, sz, p]
     return L
 
 
@@ -111,3 +112,8 @@ def test_argument_validation_without_launch(lib):
     )
     # B == 0 is a no-op
     assert lib.tlpc_lpc_forward(TLPC_F32, None, buf, None, None, 0, 8, 2, 0, 0, 0, None, 0, None) == 0
+    # an empty signal (T == 0): per-sample buffers have no storage, so null is valid
+    # there; a time-invariant A is per signal and still required
+    assert lib.tlpc_lpc_forward(TLPC_F32, None, None, buf, None, 2, 0, 4, 0, 0, 0, None, 0, None) == 0
+    assert lib.tlpc_lpc_forward(TLPC_F32, None, None, buf, None, 2, 0, 4, TI, 0, 0, None, 0, None) == TLPC_EINVAL
+    assert lib.tlpc_scan(TLPC_F32, None, None, buf, None, 2, 0, 0, None, 0, None) == 0

@@ -388,3 +388,60 @@ def test_signal_strategy_forward(T, chunk):
     y = host(ops().lpc(dev(x), dev(A), dev(zi), 2, chunk))
     ref = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
     assert rel(y, ref) <= 1e-5, rel(y, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize(
+    "B,T,N,ti",
+    [
+        (0, 100, 4, False),
+        (0, 0, 16, False),
+        (3, 0, 4, False),
+        (3, 0, 16, False),
+        (3, 0, 4, True),
+        (3, 0, 1, False),
+        (0, 50, 1, False),
+        (2, 3, 8, False),
+        (2, 5, 16, False),
+        (2, 1, 32, False),
+        (2, 7, 16, True),
+        (2, 2, 1, False),
+    ],
+)
+def test_empty_and_short_signals(dtype, B, T, N, ti):
+    """Empty batches, empty signals and T < N through sample_wise_lpc and autograd
+    (reference __init__.py:63-104: zf = y[:, -N:].flip(1) has shape (B, T) when
+    T < N). Gradients flow into y and zf together, as zf is a view of y."""
+    from torchlpc_amd import sample_wise_lpc
+
+    rng = np.random.default_rng(B * 1000 + T * 10 + N)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    x = rng.standard_normal((B, T)).astype(dtype)
+    a = rng.uniform(-0.5 / N, 0.5 / N, (B, N) if ti else (B, T, N)).astype(dtype)
+    zi = rng.standard_normal((B, N)).astype(dtype)
+    xt, at, zt = (dev(v).requires_grad_() for v in (x, a, zi))
+    y, zf = sample_wise_lpc(xt, at, zt, return_zf=True)
+    nz = min(T, N)
+    assert y.shape == (B, T) and zf.shape == (B, nz)
+    gy = rng.standard_normal((B, T)).astype(dtype)
+    gz = rng.standard_normal((B, nz)).astype(dtype)
+    torch.autograd.backward([y, zf], [dev(gy), dev(gz)])
+    assert xt.grad.shape == (B, T) and at.grad.shape == a.shape and zt.grad.shape == (B, N)
+    if B == 0 or T == 0:
+        assert y.numel() == 0 and zf.numel() == 0
+        # nothing flows back through an empty signal
+        assert not host(zt.grad).any() and not host(at.grad).any()
+        return
+    A_full = np.broadcast_to(a[:, None, :], (B, T, N)) if ti else a
+    ry = O.lpc(up(x, dtype), up(A_full, dtype), up(zi, dtype))
+    g = up(gy, dtype).copy()
+    g[:, T - nz :] += up(gz, dtype)[:, ::-1]
+    rx, rA, rzi = O.lpc_backward(g, up(A_full, dtype), up(zi, dtype), ry)
+    if ti:
+        rA = rA.sum(axis=1)
+    tol = REL_TOL[dtype]
+    assert rel(host(y), ry) <= tol
+    assert rel(host(zf), ry[:, ::-1][:, :nz]) <= tol
+    assert rel(host(xt.grad), rx) <= tol
+    assert rel(host(at.grad), rA) <= tol
+    assert rel(host(zt.grad), rzi) <= tol

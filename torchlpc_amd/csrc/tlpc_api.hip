@@ -233,10 +233,15 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   const bool need_dx = backward && (flags & TLPC_NEED_DX);
   const bool need_da = backward && (flags & TLPC_NEED_DA);
   const bool need_dzi = backward && (flags & TLPC_NEED_DZI);
-  if (!A || (!x && T > 0)) return TLPC_EINVAL;
-  if (!backward && !y && T > 0) return TLPC_EINVAL;
-  if (backward && need_da && !y && T > 0) return TLPC_EINVAL;
-  if ((need_dx && !dx) || (need_da && !dA) || (need_dzi && !dzi)) return TLPC_EINVAL;
+  // per-sample buffers of an empty signal hold no elements (torch gives them no
+  // storage); only zi, dzi and a reduced da are per signal
+  const bool none = T == 0;
+  if ((!A && !(none && !ti)) || (!x && !none)) return TLPC_EINVAL;
+  if (!backward && !y && !none) return TLPC_EINVAL;
+  if (backward && need_da && !y && !none) return TLPC_EINVAL;
+  if ((need_dx && !dx && !none) || (need_da && !dA && !(none && !da_reduced)) ||
+      (need_dzi && !dzi))
+    return TLPC_EINVAL;
   if (ws_bytes < p.total || (p.total > 0 && !ws)) return TLPC_EWORKSPACE;
 
   if (T == 0) {
