@@ -1742,6 +1742,12 @@ bool tv4_ok(const ChunkArgs<S>& a, int np, const void* y, const void* dx, const 
          al(a.x) && al(a.A) && al(y) && (!dx || al(dx)) && (!dA || al(dA));
 }
 
+// Workgroup size of the reruns (one lane per chunk). A launch of few chunks (an
+// order group of a ragged batch, a short batch) would fill few compute units
+// with 256-lane workgroups; under 64K chunks it runs 64-lane ones, which spread
+// the same lanes over four times as many.
+inline unsigned rerun_threads(int64_t nch) { return nch < 256 * 256 ? 64u : 256u; }
+
 template <typename S, int NP>
 int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   using C = ChunkCfg<S, NP>;
@@ -1756,11 +1762,12 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
     }
     hipLaunchKernelGGL((carry_ti<S, NP, true>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
-    const dim3 g((unsigned)((nch + 255) / 256));
+    const unsigned rt = rerun_threads(nch);
+    const dim3 g((unsigned)((nch + rt - 1) / rt));
     if (ti_vec(a, a.x, a.y))
-      hipLaunchKernelGGL((fwd_rerun_ti<S, NP, kF32<S>>), g, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((fwd_rerun_ti<S, NP, kF32<S>>), g, dim3(rt), 0, st, a);
     else
-      hipLaunchKernelGGL((fwd_rerun_ti<S, NP, false>), g, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((fwd_rerun_ti<S, NP, false>), g, dim3(rt), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   if (a.P > 1) {
@@ -1778,14 +1785,15 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_pipe<S, NP, true>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
-  const dim3 g((unsigned)((nch + 255) / 256));
+  const unsigned rt = rerun_threads(nch);
+  const dim3 g((unsigned)((nch + rt - 1) / rt));
   if constexpr (kTv4<S, NP>) {
     if (tv4_ok(a, NP, a.y, nullptr, nullptr)) {
-      hipLaunchKernelGGL((fwd_rerun_tv4<NP>), g, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((fwd_rerun_tv4<NP>), g, dim3(rt), 0, st, a);
       return hipGetLastError() == hipSuccess ? 0 : -4;
     }
   }
-  hipLaunchKernelGGL((fwd_rerun<S, NP>), g, dim3(256), 0, st, a);
+  hipLaunchKernelGGL((fwd_rerun<S, NP>), g, dim3(rt), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
@@ -1803,7 +1811,8 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     }
     hipLaunchKernelGGL((carry_ti<S, NP, false>), dim3((unsigned)a.B), dim3(64), 0, st, a);
     const int64_t nch = a.B * a.P;
-    const dim3 g((unsigned)((nch + 255) / 256));
+    const unsigned rt = rerun_threads(nch);
+    const dim3 g((unsigned)((nch + rt - 1) / rt));
     // with da partials the y window moves in float4 blocks too (y aligned, and
     // the zi blocks in the workspace)
     const bool vec = ti_vec(a, a.x, a.dx) &&
@@ -1812,16 +1821,16 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     // kernel, which reads the one row at stride 0
     if (a.dapart) {
       if (vec)
-        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL, kF32<S>>), g, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL, kF32<S>>), g, dim3(rt), 0, st, a);
       else
-        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL, false>), g, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_PARTIAL, false>), g, dim3(rt), 0, st, a);
     } else if (a.dA) {
-      hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(rt), 0, st, a);
     } else {
       if (vec)
-        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE, kF32<S>>), g, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE, kF32<S>>), g, dim3(rt), 0, st, a);
       else
-        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE, false>), g, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((bwd_rerun_ti<S, NP, DA_NONE, false>), g, dim3(rt), 0, st, a);
     }
     if (a.dapart) {
       const int64_t nd = a.B * a.n;
@@ -1849,15 +1858,16 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_pipe<S, NP, false>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
-  const dim3 g((unsigned)((nch + 255) / 256));
+  const unsigned rt = rerun_threads(nch);
+  const dim3 g((unsigned)((nch + rt - 1) / rt));
   bool tv4 = false;
   if constexpr (kTv4<S, NP>) {
     if (tv4_ok(a, NP, a.y, a.dx, a.dA)) {
-      hipLaunchKernelGGL((bwd_rerun_tv4<NP>), g, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((bwd_rerun_tv4<NP>), g, dim3(rt), 0, st, a);
       tv4 = true;
     }
   }
-  if (!tv4) hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(256), 0, st, a);
+  if (!tv4) hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(rt), 0, st, a);
   if (a.dapart) {
     const int64_t nd = a.B * a.n;
     hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);
