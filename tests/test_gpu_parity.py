@@ -445,3 +445,41 @@ def test_empty_and_short_signals(dtype, B, T, N, ti):
     assert rel(host(xt.grad), rx) <= tol
     assert rel(host(at.grad), rA) <= tol
     assert rel(host(zt.grad), rzi) <= tol
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("N,T", [(1, 32005), (2, 32005), (3, 65603), (4, 24600)])
+def test_long_carry_chains(dtype, N, T):
+    """Chunks of 16 samples give chains of 1 500 to 4 100 links per signal; at
+    orders up to 4 they are split into segments of about 512 links (one
+    workgroup each, a segment count that does not divide the links), forward
+    and adjoint."""
+    tol = REL_TOL[dtype]
+    y, ref = fwd_case(T + N, 2, T, N, dtype, strategy=1, chunk=16)
+    assert rel(y, ref) <= tol
+    (dx, dA, dzi), (rx, rA, rzi) = bwd_case(T + 7 * N, 2, T, N, dtype, strategy=1, chunk=16)
+    assert rel(dx, rx) <= tol and rel(dA, rA) <= tol and rel(dzi, rzi) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex128])
+def test_scan_long_signals(dtype):
+    """First-order scan of 2 signals of 2^20 samples: the library's chunk choice
+    gives chains of 16 383 links, run in 32 segments."""
+    rng = np.random.default_rng(11)
+    B, T = 2, 1 << 20
+    x = rng.standard_normal((B, T))
+    w = rng.uniform(-0.999, 0.999, (B, T))
+    init = rng.standard_normal(B)
+    g = rng.standard_normal((B, T))
+    if np.issubdtype(dtype, np.complexfloating):
+        w = w * np.exp(1j * rng.uniform(0, 2 * np.pi, (B, T)))
+        x = x + 1j * rng.standard_normal((B, T))
+    x, w, init, g = (v.astype(dtype) for v in (x, w, init, g))
+    h = ops().scan(dev(x), dev(w), dev(init))
+    h64 = O.scan(up(x, dtype), up(w, dtype), up(init, dtype))
+    assert rel(host(h), h64) <= REL_TOL[dtype]
+    dx, dw, dinit = ops().scan_backward(dev(g), dev(w), dev(init), h, True, True, True)
+    rx, rw, rinit = O.scan_backward(up(g, dtype), up(w, dtype), up(init, dtype), h64)
+    assert rel(host(dx), rx) <= REL_TOL[dtype]
+    assert rel(host(dw), rw) <= REL_TOL[dtype]
+    assert rel(host(dinit), rinit) <= REL_TOL[dtype]

@@ -132,9 +132,23 @@ struct Plan {
   int64_t P = 0;
   bool c16 = false;  // float32 order-16 time-varying kernels
   size_t summ_bytes = 0;
-  size_t summ_off = 0, carry_off = 0, wsum_off = 0, dapart_off = 0, ypad_off = 0, total = 0;
+  size_t summ_off = 0, carry_off = 0, wsum_off = 0, dapart_off = 0, ypad_off = 0, segs_off = 0,
+         total = 0;
   bool ypad = false;  // time-invariant adjoint with da partials: the y window's zi blocks
+  int64_t nseg = 0;   // segments per carry chain (0: one workgroup per chain)
 };
+
+// Segments of a long carry chain (time-varying, uniform batch, orders up to 4,
+// which take the blocked carry scan): about 512 links each, at least 4 and at
+// most 256 of them; 0 keeps one workgroup per chain. The segmented launches put
+// the signals on the grid's y axis (at most 65535).
+inline int64_t carry_segments(int np, int64_t B, int64_t P, bool ti, bool c16) {
+  if (ti || c16 || np > 4 || B > 65535) return 0;
+  const int64_t links = P - 1;
+  const int64_t nseg = (links + 511) / 512;
+  if (nseg < 4) return 0;
+  return nseg > 256 ? 256 : nseg;
+}
 
 // The order-16 float32 kernels serve time-varying filters with chunks that are
 // whole 16-step blocks and 32-bit sample indices inside a signal.
@@ -187,7 +201,9 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   p.dapart_off = p.wsum_off + align_up(wsum);
   p.ypad = dapart > 0;
   p.ypad_off = p.dapart_off + align_up(dapart);
-  p.total = p.ypad_off + (p.ypad ? align_up((size_t)B * ti_ypad(p.np) * es) : 0);
+  p.segs_off = p.ypad_off + (p.ypad ? align_up((size_t)B * ti_ypad(p.np) * es) : 0);
+  p.nseg = carry_segments(p.np, B, p.P, ti, p.c16);
+  p.total = p.segs_off + align_up((size_t)B * p.nseg * (p.np * p.np + 2 * p.np) * ws_es);
   return p;
 }
 
@@ -282,6 +298,10 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   a.summ = reinterpret_cast<W*>(w + p.summ_off);
   a.carry = reinterpret_cast<W*>(w + p.carry_off);
   a.wsum = reinterpret_cast<W*>(w + p.wsum_off);
+  if (p.nseg > 0) {
+    a.segs = reinterpret_cast<W*>(w + p.segs_off);
+    a.nseg = p.nseg;
+  }
   if (need_da && da_reduced) {
     a.dapart = reinterpret_cast<W*>(w + p.dapart_off);
     if (p.ypad) a.ypad = reinterpret_cast<S*>(w + p.ypad_off);

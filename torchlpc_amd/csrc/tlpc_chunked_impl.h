@@ -489,18 +489,35 @@ __device__ __forceinline__ void load_affine(Affine<S, NP>& f, const S* sm) {
 
 constexpr int kBlockedThreads = 256;
 
-template <typename S, int NP, bool FWD>
+// Long chains of a uniform batch are split into a.nseg segments of links, one
+// workgroup each: the per-link loads of one workgroup are scattered (a thread's
+// run of links is contiguous, so a wave's loads touch 64 lines each), and a
+// signal's whole chain in one workgroup was bound by that one CU's memory
+// pipeline. SEG_REDUCE composes a segment's links into one map, carry_segs
+// chains the maps of each signal, and SEG_APPLY re-walks every segment from its
+// incoming state. SEG_WHOLE walks a signal's chain in one workgroup.
+// Workspace a.segs: the maps [B][nseg][NP*NP+NP], then the incoming states
+// [B][nseg][NP].
+enum { SEG_WHOLE = 0, SEG_REDUCE = 1, SEG_APPLY = 2 };
+
+template <typename S, int NP, bool FWD, int MODE>
 __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a) {
   using W = typename Wide<S>::type;
   __shared__ Affine<W, NP> sh[kBlockedThreads];
-  const int64_t b = blockIdx.x;
+  const int64_t b = MODE == SEG_WHOLE ? blockIdx.x : blockIdx.y;
   const int tid = threadIdx.x;
   const Span sp = sig_span(a, b);
   const int64_t P = sp.P;
   const int64_t links = P - 1;  // number of affine maps in the chain
-  const int64_t per = (links + kBlockedThreads - 1) / kBlockedThreads;
-  const int64_t lo = tid * per;
-  const int64_t hi = (lo + per < links) ? lo + per : links;
+  int64_t l0 = 0, l1 = links;   // this workgroup's links
+  if constexpr (MODE != SEG_WHOLE) {
+    const int64_t seg = (links + a.nseg - 1) / a.nseg;
+    l0 = min((int64_t)blockIdx.x * seg, links);
+    l1 = min(l0 + seg, links);
+  }
+  const int64_t per = (l1 - l0 + kBlockedThreads - 1) / kBlockedThreads;
+  const int64_t lo = min(l0 + tid * per, l1);
+  const int64_t hi = min(lo + per, l1);
   // link l (0-based, in processing order) maps carry of step l to carry of step l+1;
   // forward: chunk c = l; adjoint: chunk c = P-1-l.
   auto link = [&](int64_t l) -> const W* {
@@ -534,11 +551,30 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
     }
     __syncthreads();
   }
-  // exclusive prefix applied to the initial carry
-  W s[NP];
+  if constexpr (MODE == SEG_REDUCE) {
+    // the last thread's inclusive composition is the segment's map
+    if (tid == kBlockedThreads - 1) {
+      W* out = a.segs + (b * a.nseg + blockIdx.x) * (NP * NP + NP);
 #pragma unroll
-  for (int r = 0; r < NP; ++r)
-    s[r] = (FWD && r < a.n && a.zi) ? convert<W>(a.zi[sp.zo + r]) : Num<W>::zero();
+      for (int r = 0; r < NP; ++r) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) out[r * NP + k] = f.M[r][k];
+        out[NP * NP + r] = f.v[r];
+      }
+    }
+    return;
+  }
+  // exclusive prefix applied to the initial carry (the segment's incoming state)
+  W s[NP];
+  if constexpr (MODE == SEG_APPLY) {
+    const W* in = a.segs + a.B * a.nseg * (NP * NP + NP) + (b * a.nseg + blockIdx.x) * NP;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) s[r] = in[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < NP; ++r)
+      s[r] = (FWD && r < a.n && a.zi) ? convert<W>(a.zi[sp.zo + r]) : Num<W>::zero();
+  }
   if (tid > 0) {
     const Affine<W, NP>& pre = sh[tid - 1];
     W t[NP];
@@ -581,6 +617,57 @@ __global__ __launch_bounds__(kBlockedThreads) void carry_blocked(ChunkArgs<S> a)
     const int64_t c = FWD ? P - 1 : 0;
 #pragma unroll
     for (int r = 0; r < NP; ++r) carry[c * NP + r] = s[r];
+  }
+}
+
+// Segmented carry, middle step: per signal, the state entering each segment
+// (one workgroup per signal; thread s loads the map of segment s, then thread 0
+// walks the maps in LDS). At most kBlockedThreads segments.
+template <typename S, int NP, bool FWD>
+__global__ __launch_bounds__(kBlockedThreads) void carry_segs(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
+  __shared__ Affine<W, NP> sh[kBlockedThreads];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const W* maps = a.segs + b * a.nseg * (NP * NP + NP);
+  if (tid < a.nseg) load_affine(sh[tid], maps + tid * (NP * NP + NP));
+  __syncthreads();
+  if (tid != 0) return;
+  const Span sp = sig_span(a, b);
+  W s[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r)
+    s[r] = (FWD && r < a.n && a.zi) ? convert<W>(a.zi[sp.zo + r]) : Num<W>::zero();
+  W* in = a.segs + a.B * a.nseg * (NP * NP + NP) + b * a.nseg * NP;
+  for (int64_t g = 0; g < a.nseg; ++g) {
+#pragma unroll
+    for (int r = 0; r < NP; ++r) in[g * NP + r] = s[r];
+    W t[NP];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+      W acc = sh[g].v[r];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) acc = Num<W>::fma(acc, sh[g].M[r][k], s[k]);
+      t[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < NP; ++r) s[r] = t[r];
+  }
+}
+
+// Launches the carry chain of every signal with the blocked scan: segmented
+// when the plan gave a segment workspace (a.nseg > 0), else one workgroup per
+// signal.
+template <typename S, int NP, bool FWD>
+void launch_carry_blocked(const ChunkArgs<S>& a, hipStream_t st) {
+  if (a.nseg > 0) {
+    const dim3 g((unsigned)a.nseg, (unsigned)a.B);
+    hipLaunchKernelGGL((carry_blocked<S, NP, FWD, SEG_REDUCE>), g, dim3(kBlockedThreads), 0, st, a);
+    hipLaunchKernelGGL((carry_segs<S, NP, FWD>), dim3((unsigned)a.B), dim3(kBlockedThreads), 0, st, a);
+    hipLaunchKernelGGL((carry_blocked<S, NP, FWD, SEG_APPLY>), g, dim3(kBlockedThreads), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((carry_blocked<S, NP, FWD, SEG_WHOLE>), dim3((unsigned)a.B), dim3(kBlockedThreads), 0,
+                       st, a);
   }
 }
 
@@ -1778,7 +1865,7 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
   bool blocked = false;
   if constexpr (blocked_fits<S, NP>()) {
     if (a.P > 64) {
-      hipLaunchKernelGGL((carry_blocked<S, NP, true>), dim3((unsigned)a.B), dim3(kBlockedThreads), 0, st, a);
+      launch_carry_blocked<S, NP, true>(a, st);
       blocked = true;
     }
   }
@@ -1851,7 +1938,7 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   bool blocked = false;
   if constexpr (blocked_fits<S, NP>()) {
     if (a.P > 64) {
-      hipLaunchKernelGGL((carry_blocked<S, NP, false>), dim3((unsigned)a.B), dim3(kBlockedThreads), 0, st, a);
+      launch_carry_blocked<S, NP, false>(a, st);
       blocked = true;
     }
   }

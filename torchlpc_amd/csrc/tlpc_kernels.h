@@ -45,6 +45,11 @@ struct ChunkArgs {
   // window's values below position 0 (y_{-1-k} = zi[k], zero past the order)
   // as ti_ypad(NP) floats ending at position -1, 16-byte aligned; or null.
   S* ypad;
+  // Long carry chains of a uniform batch at low orders: nseg segments per signal
+  // (0: one workgroup walks each chain), with their maps and incoming states in
+  // segs (workspace).
+  typename Wide<S>::type* segs;
+  int64_t nseg;
 };
 
 // Floats per signal of ChunkArgs::ypad: the y window of an order-NP adjoint
