@@ -55,7 +55,6 @@ template <typename S, int NP, int LANES = NP + 1>
 struct ItemTab {
   static constexpr int G = ChunkCfg<S, NP, LANES>::G;
   int64_t a0[G];   // offset of A at the chunk's first sample
-  int64_t at[G];   // A stride over time
   int64_t x0[G];   // offset of the chunk's first input sample
   int64_t rem[G];  // samples of the signal from the chunk start (<= 0: idle)
 };
@@ -89,9 +88,55 @@ __device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP,
   if (fwd ? (c >= sp.P - 1) : (c < 1)) return;
   const int64_t t = c * a.L;
   tab.a0[gg] = sp.ao + t * sp.at;
-  tab.at[gg] = sp.at;
   tab.x0[gg] = sp.xo + t;
   tab.rem[gg] = sp.T - t;
+}
+
+// Staging: a thread stages the same elements (item, step in the block, lag) in
+// every time block of its items, so their addresses are worked out once
+// (stage_map): the offset of each element in the block at t0 = 0, and `lim`,
+// the element being inside its signal (and its lag below n) while t0 < lim.
+template <typename S, int NP, int LANES>
+struct StageMap {
+  using C = ChunkCfg<S, NP, LANES>;
+  static constexpr int NT = C::THREADS;
+  static constexpr int NA = C::G * C::TB * NP;
+  static constexpr int NX = C::G * C::TB;
+  static constexpr int ITA = (NA + NT - 1) / NT;
+  static constexpr int ITX = (NX + NT - 1) / NT;
+  int64_t pa[ITA], px[ITX];
+  int lima[ITA], limx[ITX];
+};
+
+template <typename S, int NP, int LANES>
+__device__ __forceinline__ void stage_map(const ChunkArgs<S>& a, StageMap<S, NP, LANES>& m,
+                                          const ItemTab<S, NP, LANES>& tab) {
+  using C = ChunkCfg<S, NP, LANES>;
+  using SM = StageMap<S, NP, LANES>;
+  constexpr int NT = SM::NT, NA = SM::NA, NX = SM::NX;
+  const int64_t at = a.rg ? a.n : a.a_tstride;  // A's time stride, the same for every signal
+#pragma unroll
+  for (int it = 0; it < SM::ITA; ++it) {
+    const int idx = it * NT + (int)threadIdx.x;
+    const int gg = min(idx, NA - 1) / (C::TB * NP);
+    const int r = min(idx, NA - 1) - gg * (C::TB * NP);
+    const int tb = r / NP;
+    const int i = r - tb * NP;
+    const int64_t rem = tab.rem[gg];
+    const bool ok = idx < NA && i < a.n && rem > tb;
+    m.lima[it] = ok ? (int)min(rem - tb, (int64_t)a.L) : 0;
+    m.pa[it] = ok ? tab.a0[gg] + tb * at + i : 0;
+  }
+#pragma unroll
+  for (int it = 0; it < SM::ITX; ++it) {
+    const int idx = it * NT + (int)threadIdx.x;
+    const int gg = min(idx, NX - 1) / C::TB;
+    const int tb = min(idx, NX - 1) - gg * C::TB;
+    const int64_t rem = tab.rem[gg];
+    const bool ok = idx < NX && rem > tb;
+    m.limx[it] = ok ? (int)min(rem - tb, (int64_t)a.L) : 0;
+    m.px[it] = ok ? tab.x0[gg] + tb : 0;
+  }
 }
 
 // Staging helper: copy the A rows and the per-step input of the G items for the
@@ -103,7 +148,7 @@ __device__ __forceinline__ void fill_items(const ChunkArgs<S>& a, ItemTab<S, NP,
 template <typename S, int NP, int LANES>
 __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide<S>::type* sA,
                                             typename Wide<S>::type* sX, const S* inp,
-                                            const ItemTab<S, NP, LANES>& tab, int t0) {
+                                            const StageMap<S, NP, LANES>& m, int t0) {
   using C = ChunkCfg<S, NP, LANES>;
   using W = typename Wide<S>::type;
   constexpr int NT = C::THREADS;
@@ -113,24 +158,17 @@ __device__ __forceinline__ void stage_block(const ChunkArgs<S>& a, typename Wide
   constexpr int ITX = (NX + NT - 1) / NT;
   S va[ITA], vx[ITX];
   bool oka[ITA], okx[ITX];
+  const int64_t toff = (int64_t)t0 * (a.rg ? a.n : a.a_tstride);
 #pragma unroll
   for (int it = 0; it < ITA; ++it) {
-    const int idx = it * NT + (int)threadIdx.x;
-    const int gg = min(idx, NA - 1) / (C::TB * NP);
-    const int r = min(idx, NA - 1) - gg * (C::TB * NP);
-    const int tb = r / NP;
-    const int i = r - tb * NP;
-    const bool ok = idx < NA && t0 + tb < tab.rem[gg] && i < a.n;
-    va[it] = a.A[ok ? tab.a0[gg] + (t0 + tb) * tab.at[gg] + i : 0];
+    const bool ok = t0 < m.lima[it];
+    va[it] = a.A[ok ? m.pa[it] + toff : 0];
     oka[it] = ok;
   }
 #pragma unroll
   for (int it = 0; it < ITX; ++it) {
-    const int idx = it * NT + (int)threadIdx.x;
-    const int gg = min(idx, NX - 1) / C::TB;
-    const int tb = min(idx, NX - 1) - gg * C::TB;
-    const bool ok = idx < NX && t0 + tb < tab.rem[gg];
-    vx[it] = inp[ok ? tab.x0[gg] + t0 + tb : 0];
+    const bool ok = t0 < m.limx[it];
+    vx[it] = inp[ok ? m.px[it] + t0 : 0];
     okx[it] = ok;
   }
 #pragma unroll
@@ -168,6 +206,8 @@ __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
   const int64_t first = (int64_t)blockIdx.x * C::G;
   fill_items<S, NP, NP>(a, tab, first, true);
   __syncthreads();
+  StageMap<S, NP, NP> sm;
+  stage_map<S, NP, NP>(a, sm, tab);
   const int g = threadIdx.x / NP;
   const int k = threadIdx.x - g * NP;
   const bool active = g < C::G && tab.rem[g] > 0;
@@ -178,7 +218,7 @@ __global__ __launch_bounds__(256) void fwd_summary(ChunkArgs<S> a) {
   W z = Num<W>::zero();
 
   for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
-    stage_block<S, NP, NP>(a, sA, sX, a.x, tab, t0);
+    stage_block<S, NP, NP>(a, sA, sX, a.x, sm, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
@@ -217,6 +257,8 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   const int64_t first = (int64_t)blockIdx.x * C::G;
   fill_items<S, NP, NP + 1>(a, tab, first, false);
   __syncthreads();
+  StageMap<S, NP, NP + 1> sm;
+  stage_map<S, NP, NP + 1>(a, sm, tab);
   const int g = threadIdx.x / (NP + 1);
   const int k = threadIdx.x - g * (NP + 1);
   const bool active = g < C::G && tab.rem[g] > 0;
@@ -231,7 +273,7 @@ __global__ __launch_bounds__(256) void bwd_summary(ChunkArgs<S> a) {
   const W gmask = (k == NP) ? Num<W>::one() : Num<W>::zero();
 
   for (int t0 = a.L - C::TB; t0 >= 0; t0 -= C::TB) {
-    stage_block<S, NP, NP + 1>(a, sA, sX, a.x, tab, t0);
+    stage_block<S, NP, NP + 1>(a, sA, sX, a.x, sm, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
@@ -284,6 +326,8 @@ __global__ __launch_bounds__(256) void bwd_summary_ff(ChunkArgs<S> a) {
   const int64_t first = (int64_t)blockIdx.x * C::G;
   fill_items<S, NP, NP>(a, tab, first, false);
   __syncthreads();
+  StageMap<S, NP, NP> sm;
+  stage_map<S, NP, NP>(a, sm, tab);
   const int g = threadIdx.x / NP;
   const int r = threadIdx.x - g * NP;
   const bool active = g < C::G && tab.rem[g] > 0;
@@ -294,7 +338,7 @@ __global__ __launch_bounds__(256) void bwd_summary_ff(ChunkArgs<S> a) {
   W w = Num<W>::zero();
 
   for (int t0 = 0; t0 < a.L; t0 += C::TB) {
-    stage_block<S, NP, NP>(a, sA, sX, a.x, tab, t0);
+    stage_block<S, NP, NP>(a, sA, sX, a.x, sm, t0);
     __syncthreads();
     if (active) {
       const W* rowbase = sA + g * C::ASTRIDE;
