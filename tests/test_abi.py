@@ -75,6 +75,20 @@ def test_workspace_queries(lib):
     assert lib.tlpc_lpc_forward_workspace(7, B, T, N, 0, 0, 0) == 0
     assert lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, 0, 0, 0, 0) == 0
     assert lib.tlpc_scan_workspace(TLPC_F32, 4, 1 << 20, 0, 0) > 0
+    # chains of more than 1536 links at orders <= 4 add segment maps and states:
+    # B * nseg * (N*N + 2N) wide elements, nseg = ceil(links / 512)
+    B, T, N, L = 4, 16 * 4100, 2, 16
+    P = T // L
+    nseg = -(-(P - 1) // 512)
+    summ = B * (P - 1) * (N * N + N) * 8
+    carry = B * P * N * 8
+    segs = B * nseg * (N * N + 2 * N) * 8
+    up = lambda v: -(-v // 256) * 256
+    assert lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, N, 0, 1, L) == up(summ) + up(carry) + up(segs)
+    # shorter chains (1536 links) keep one workgroup per chain and no segment space
+    T = 16 * 1537
+    P = T // L
+    assert lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, N, 0, 1, L) == up(B * (P - 1) * 6 * 8) + up(B * P * 2 * 8)
 
 
 def test_summary_bytes(lib):
