@@ -95,13 +95,16 @@ inline int pick_chunk16(int64_t B, int64_t T, int requested) {
 }
 
 // Time-invariant chunks: the summary cost no longer grows with the order (Phi is
-// built once per signal), so chunks are short enough for about 128K lanes in the
+// built once per signal), so chunks are short enough for about 48K lanes in the
 // zero-state and rerun phases, at least 4*NP long, and at most 256 per signal
-// (the length of the carry chain). Summaries and carries are in the wide type,
+// (the length of the carry chain). More lanes hide more of the reruns' latency
+// but lengthen the carry chains: at B=1024, T=16384, N=32 a step took 0.475 ms
+// with chunks of 128 samples (128K lanes), 0.357 / 0.350 ms at 320 / 384 and
+// 0.388 ms at 512. Summaries and carries are in the wide type,
 // so short float32 chunks keep their accuracy.
 inline int pick_chunk_ti(int64_t B, int64_t T, int np, int requested) {
   const int blk = chunk_block(np);
-  int64_t L = requested > 0 ? requested : (B * T + 131071) / 131072;
+  int64_t L = requested > 0 ? requested : (B * T + 49151) / 49152;
   if (requested <= 0) {
     if (L < 4 * np) L = 4 * np;
     if (L < (T + 255) / 256) L = (T + 255) / 256;
