@@ -140,6 +140,34 @@ def cpu_baseline(T, N, sample_signals=64, min_seconds=10.0):
     }
 
 
+def accuracy_sample(x, A, zi, w, outs, signals):
+    """Relative L2 errors of one timed step's outputs against the FP64 oracle, on a
+    subset of the batch's signals (signals are independent, so the rows of the
+    full-batch outputs are checked as they are)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    idx = list(signals)
+    h = lambda t: t[idx].detach().cpu().numpy().astype(np.float64)  # noqa: E731
+    y, (dx, dA, dzi) = outs
+    x64, A64, zi64, w64 = h(x), h(A), h(zi), h(w)
+    y64 = O.lpc(x64, A64, zi64)
+    rx, rA, rzi = O.lpc_backward(w64, A64, zi64, y64)
+    N = A64.shape[-1]
+    yo = y[idx].detach().cpu().numpy()
+    return {
+        "y": O.rel_err(yo, y64),
+        "zf": O.rel_err(yo[:, ::-1][:, :N], y64[:, ::-1][:, :N]),
+        "dx": O.rel_err(h(dx), rx),
+        "dA": O.rel_err(h(dA), rA),
+        "dzi": O.rel_err(h(dzi), rzi),
+        "signals": idx,
+        "bar": 1e-5,
+        "reference": "FP64 oracle (C restatement of the reference loop) on the float32 inputs",
+    }
+
+
 def _cpu_model():
     """The host CPU's model name (Linux /proc/cpuinfo), or "unknown"."""
     try:
@@ -176,6 +204,53 @@ def _interp(anchors, T, hop):
     hi = (lo + 1).clamp(max=n_anchor - 1)
     frac = (pos - lo)[:, None]
     return anchors[..., lo, :] * (1 - frac) + anchors[..., hi, :] * frac
+
+
+def ragged512_inputs(device, K=512, seed=1004, hop=256):
+    """K ragged signals: lengths ~ U{1024..65536}, orders uniform over {2,4,8,16,32},
+    packed in order of their filter order (all order-2 signals first, and so on),
+    so that each order's samples, coefficient rows and states are one contiguous
+    slice. Coefficients come from reflection coefficients k ~ U(-0.9, 0.9) at
+    anchors every `hop` samples (U(-0.5, 0.5) at order 32, where 0.9 makes the
+    time-varying filter grow past the float32 range), interpolated per sample and
+    converted by the step-up recursion. Each order is generated in a few batched
+    device ops (about 20 plus 5 per order of the step-up), not per signal.
+    Returns (lengths, orders, x, A, zi) with float64 device tensors."""
+    import torch
+
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    lengths = torch.randint(1024, 65537, (K,), generator=g)
+    orders = torch.tensor((2, 4, 8, 16, 32))[torch.randint(0, 5, (K,), generator=g)]
+    perm = torch.sort(orders, stable=True).indices
+    lengths, orders = lengths[perm], orders[perm]
+    gd = torch.Generator(device=device).manual_seed(seed)
+    f64 = dict(device=device, dtype=torch.float64)
+    x = torch.randn(int(lengths.sum()), generator=gd, **f64)
+    zi = torch.randn(int(orders.sum()), generator=gd, **f64) * 0.1
+    parts = []
+    for n in (2, 4, 8, 16, 32):
+        sel = (orders == n).nonzero().flatten()
+        if sel.numel() == 0:
+            continue
+        lens = lengths[sel].to(device)
+        nanc = (lens - 1) // hop + 2
+        kmax = 0.9 if n <= 16 else 0.5
+        anchors = (torch.rand(int(nanc.sum()), n, generator=gd, **f64) * 2 - 1) * kmax
+        zero = torch.zeros(1, dtype=torch.int64, device=device)
+        abase = torch.cat([zero, nanc.cumsum(0)[:-1]])
+        start = torch.cat([zero, lens.cumsum(0)[:-1]])
+        sig = torch.repeat_interleave(torch.arange(sel.numel(), device=device), lens)
+        t = torch.arange(sig.numel(), device=device) - start[sig]
+        pos = t.to(torch.float64) / hop
+        last = nanc[sig] - 1
+        lo = torch.minimum(pos.floor().long(), last)
+        hi = torch.minimum(lo + 1, last)
+        frac = (pos - lo)[:, None]
+        k = anchors[abase[sig] + lo] * (1 - frac) + anchors[abase[sig] + hi] * frac
+        del anchors, sig, t, pos, lo, hi, frac
+        parts.append(_step_up(k).reshape(-1))
+        del k
+    return lengths.tolist(), orders.tolist(), x, torch.cat(parts), zi
 
 
 def workload_inputs(name, device, seed):
@@ -233,19 +308,10 @@ def bench_workload(name, steps, warmup, ragged_order=0):
     stream = torch.cuda.current_stream()
     f32 = torch.float32
     if name == "ragged512":  # 512 ragged signals, lengths ~ U{1024..65536}, orders in {2,4,8,16,32}
-        g = torch.Generator(device="cpu").manual_seed(1004)
         K = 512
-        lengths = torch.randint(1024, 65537, (K,), generator=g).tolist()
-        orders = [(2, 4, 8, 16, 32)[i] for i in torch.randint(0, 5, (K,), generator=g).tolist()]
-        gd = torch.Generator(device=dev).manual_seed(1004)
-        xs, As, zs = [], [], []
-        for T, N in zip(lengths, orders):
-            kmax = 0.9 if N <= 16 else 0.5  # order 32 with 0.9 grows past float32
-            kk = (torch.rand(1, (T - 1) // 256 + 2, N, generator=gd, device=dev, dtype=torch.float64) * 2 - 1) * kmax
-            As.append(_step_up(_interp(kk, T, 256))[0].reshape(-1))
-            xs.append(torch.randn(T, generator=gd, device=dev, dtype=torch.float64))
-            zs.append(torch.randn(N, generator=gd, device=dev, dtype=torch.float64) * 0.1)
-        x, A, zi = (torch.cat(v).to(f32) for v in (xs, As, zs))
+        lengths, orders, x, A, zi = ragged512_inputs(dev, K)
+        x, A, zi = x.to(f32), A.to(f32), zi.to(f32)
+        gd = torch.Generator(device=dev).manual_seed(1014)
         w = torch.randn(x.numel(), generator=gd, device=dev, dtype=f32)
         plan = torchlpc_amd.RaggedPlan(lengths, orders, dev, f32)
         y = torch.empty_like(x)
@@ -628,6 +694,13 @@ def main():
         log(f"[bench] plain GPU form {plain_ms:.3f} ms  (fused speedup {plain_ms / ms:.2f}x)")
 
     if not args.no_cpu_baseline:
+        log("[bench] accuracy of the timed plan on a subset of signals")
+        outs = step()
+        torch.cuda.synchronize()
+        sig = sorted({round(i * (B - 1) / 7) for i in range(8)})
+        result["rel_err"] = accuracy_sample(x, A, zi, w, outs, sig)
+        del outs
+        log(f"[bench] rel_err {result['rel_err']}")
         log("[bench] reference CPU path")
         result["cpu_baseline"] = cpu_baseline(T, N)
         log(f"[bench] cpu {result['cpu_baseline']['value']:.3f} Msamples/s")

@@ -92,7 +92,7 @@ static inline double complex conjc128_(double complex v) { return conj(v); }
     T_* rA = (T_*)calloc((size_t)(B * S * N), sizeof(T_));                                 \
     T_* z0 = (T_*)calloc((size_t)(B * N), sizeof(T_));                                     \
     T_* out = (T_*)malloc(sizeof(T_) * (size_t)(B * S));                                   \
-    for (int64_t b = 0; b < B; ++b) {                                                      \
+    _Pragma("omp parallel for") for (int64_t b = 0; b < B; ++b) {                          \
       /* padded gradient [0]*N ++ g, flipped: rg[u] = g[T-1-u] for u < T */               \
       for (int64_t u = 0; u < T; ++u) rg[b * S + u] = g[b * T + (T - 1 - u)];              \
       /* shifted[s, i-1] = A[s-N+i, i-1]; reversed in time and conjugated */               \
@@ -104,7 +104,7 @@ static inline double complex conjc128_(double complex v) { return conj(v); }
         }                                                                                  \
     }                                                                                      \
     LPC(rg, rA, z0, out, B, S, N);                                                         \
-    for (int64_t b = 0; b < B; ++b) {                                                      \
+    _Pragma("omp parallel for") for (int64_t b = 0; b < B; ++b) {                          \
       const T_* o = out + b * S;                                                           \
       if (dzi)                                                                             \
         for (int64_t k = 0; k < N; ++k) dzi[b * N + k] = o[T + k];                         \

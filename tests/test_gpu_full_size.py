@@ -64,6 +64,35 @@ def test_time_invariant_b1024_t16384_n32_full_size():
     assert max(errs) <= 1e-5, errs
 
 
+def test_reflection_b256_t48000_n16_timed_plan():
+    """The B=256, T=48000, N=16 step exactly as bench.py times it: the same seeded
+    inputs (generated on the GPU), the automatic plan, lpc_keep and then
+    lpc_backward reusing the kept workspace. y, zf, dx, dA and dzi of all 256
+    signals within 1e-5 (relative L2) of the FP64 oracle."""
+    import bench
+
+    B, T, N = 256, 48000, 16
+    x, A, zi, w = bench.make_inputs(B, T, N, torch.device("cuda:0"))
+    y, ws = ops().lpc_keep(x, A, zi, 0, 0)
+    assert ws.numel() > 0, "the forward kept no workspace for the backward"
+    dx, dA, dzi = ops().lpc_backward(w, A, zi, y, True, True, True, 0, 0, ws)
+    torch.cuda.synchronize()
+    up = lambda t: host(t).astype(np.float64)  # noqa: E731
+    x64, A64, zi64, w64 = up(x), up(A), up(zi), up(w)
+    del x, A, zi, w, ws
+    y64 = O.lpc(x64, A64, zi64)
+    rx, rA, rzi = O.lpc_backward(w64, A64, zi64, y64)
+    errs = {
+        "y": rel(host(y), y64),
+        "zf": rel(host(y)[:, ::-1][:, :N], y64[:, ::-1][:, :N]),
+        "dx": rel(host(dx), rx),
+        "dA": rel(host(dA), rA),
+        "dzi": rel(host(dzi), rzi),
+    }
+    print("b256-t48000-n16 rel errors", errs, flush=True)
+    assert max(errs.values()) <= 1e-5, errs
+
+
 @pytest.mark.parametrize("N,T,B", [(16, 5000, 3), (16, 3000, 66), (8, 4096, 3), (4, 3000, 3), (2, 70000, 3), (32, 2001, 2)])
 def test_hip_graph_capture_replays(N, T, B):
     """The forward and the fused backward captured in one HIP graph: every replay
