@@ -43,6 +43,8 @@ extern "C" {
 #define TLPC_EWORKSPACE -3 /* workspace smaller than the matching *_workspace() query */
 #define TLPC_ELAUNCH -4    /* hipGetLastError() reported a launch failure */
 #define TLPC_EORDER -5     /* unsupported filter order */
+#define TLPC_ESCRATCH -6   /* tlpc_lpc_forward_noalloc: sizes need more than its static scratch */
+#define TLPC_EDEVICE -7    /* no current HIP device, or its scratch address could not be read */
 
 /* flags for the backward entry points: which gradients to write */
 #define TLPC_NEED_DX 1
@@ -81,6 +83,26 @@ size_t tlpc_lpc_forward_workspace(int dtype, int64_t B, int64_t T, int32_t N, in
 int tlpc_lpc_forward(int dtype, const void* x, const void* A, const void* zi, void* y, int64_t B,
                      int64_t T, int32_t N, int32_t flags, int32_t strategy, int32_t chunk_len,
                      void* workspace, size_t workspace_bytes, void* hip_stream);
+
+/*
+ * Allocation-free forward: tlpc_lpc_forward with the workspace taken from a
+ * static device array of TLPC_NOALLOC_SCRATCH_BYTES per GPU instead of the
+ * caller. No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc / hipFree and no
+ * allocating library routine. Sizes whose tlpc_lpc_forward_workspace() exceeds
+ * TLPC_NOALLOC_SCRATCH_BYTES are rejected with TLPC_ESCRATCH before anything is
+ * launched. The array is held by every GPU the library runs on until the
+ * process ends, and every call on one GPU shares it: calls that overlap on one
+ * GPU (concurrent streams, or host threads) race and must be serialised by the
+ * caller. Nothing in it needs to start at zero (every kernel writes the scratch
+ * it reads within the same call). The first call on a device, with that device
+ * current, reads the array's device address (hipGetSymbolAddress) and keeps it.
+ */
+#define TLPC_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tlpc_lpc_forward_noalloc(int dtype, const void* x, const void* A, const void* zi, void* y,
+                             int64_t B, int64_t T, int32_t N, int32_t flags, int32_t strategy,
+                             int32_t chunk_len, void* hip_stream);
 
 /*
  * Fused adjoint. Replaces the backward composition of LPC.backward

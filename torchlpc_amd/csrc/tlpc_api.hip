@@ -3,10 +3,16 @@
 #include <stddef.h>
 
 #include <atomic>
+#include <mutex>
 #include <stdint.h>
 
 #include "../../include/torchlpc_amd.h"
 #include "tlpc_kernels.h"
+
+// Workspace of tlpc_lpc_forward_noalloc: one copy per GPU, held until the
+// process ends. In host code `scratch` is only a stand-in for the symbol; the
+// device address comes from hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[TLPC_NOALLOC_SCRATCH_BYTES];
 
 namespace tlpc {
 namespace {
@@ -213,6 +219,25 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
 // Ragged workspace: summaries and carries indexed by chunk id.
 inline size_t ragged_workspace(size_t wsz, int64_t G, int np) {
   return align_up((size_t)G * (np * np + np) * wsz) + align_up((size_t)G * np * wsz);
+}
+
+// Device address of `scratch` on the current device, read once per device (with
+// that device current) and kept; later calls return the kept address.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_dev[kMaxDevices];
+
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return TLPC_EDEVICE;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TLPC_EDEVICE;
+    scratch_dev[dev] = p;
+  }
+  *out = scratch_dev[dev];
+  return TLPC_OK;
 }
 
 int check_common(int dtype, int64_t B, int64_t T, int32_t n) {
@@ -437,6 +462,8 @@ TLPC_EXPORT const char* tlpc_error_string(int code) {
     case TLPC_EWORKSPACE: return "workspace too small";
     case TLPC_ELAUNCH: return "kernel launch failed";
     case TLPC_EORDER: return "unsupported filter order";
+    case TLPC_ESCRATCH: return "sizes exceed the static scratch of tlpc_lpc_forward_noalloc";
+    case TLPC_EDEVICE: return "no current HIP device, or its scratch address is unavailable";
     default: return "unknown error";
   }
 }
@@ -456,6 +483,23 @@ TLPC_EXPORT int tlpc_lpc_forward(int dtype, const void* x, const void* A, const 
   return run_dtype(dtype, x, A, zi, y, nullptr, nullptr, nullptr, B, T, N,
                    flags & TLPC_TIME_INVARIANT, strategy, chunk_len, workspace, workspace_bytes,
                    hip_stream, false);
+}
+
+TLPC_EXPORT int tlpc_lpc_forward_noalloc(int dtype, const void* x, const void* A, const void* zi,
+                                         void* y, int64_t B, int64_t T, int32_t N, int32_t flags,
+                                         int32_t strategy, int32_t chunk_len, void* hip_stream) {
+  int rc = check_common(dtype, B, T, N);
+  if (rc != TLPC_OK) return rc;
+  const int32_t f = flags & TLPC_TIME_INVARIANT;
+  const size_t need = make_plan(dtype, B, T, N, f, strategy, chunk_len, false).total;
+  if (need > TLPC_NOALLOC_SCRATCH_BYTES) return TLPC_ESCRATCH;
+  void* ws = nullptr;
+  if (need > 0) {
+    rc = scratch_address(&ws);
+    if (rc != TLPC_OK) return rc;
+  }
+  return run_dtype(dtype, x, A, zi, y, nullptr, nullptr, nullptr, B, T, N, f, strategy, chunk_len,
+                   ws, need > 0 ? TLPC_NOALLOC_SCRATCH_BYTES : 0, hip_stream, false);
 }
 
 TLPC_EXPORT size_t tlpc_lpc_backward_workspace(int dtype, int64_t B, int64_t T, int32_t N,
