@@ -65,7 +65,7 @@ extern "C" {
 /* strategy for tlpc_lpc_forward / tlpc_lpc_backward (`strategy` argument) */
 #define TLPC_STRATEGY_AUTO 0    /* library heuristic */
 #define TLPC_STRATEGY_CHUNKED 1 /* multi-kernel chunked scan (any B) */
-#define TLPC_STRATEGY_SIGNAL 2  /* one workgroup per signal, fused tile pipeline */
+#define TLPC_STRATEGY_SIGNAL 2  /* one workgroup per signal walking LDS tiles (float32, N = 16) */
 #define TLPC_STRATEGY_SERIAL 3  /* one thread per signal, no time parallelism */
 
 const char* tlpc_error_string(int code);
@@ -120,6 +120,17 @@ int tlpc_lpc_backward(int dtype, const void* g, const void* A, const void* zi, c
                       void* dx, void* dA, void* dzi, int64_t B, int64_t T, int32_t N,
                       int32_t flags, int32_t strategy, int32_t chunk_len, void* workspace,
                       size_t workspace_bytes, void* hip_stream);
+
+/*
+ * The plan that tlpc_lpc_forward (backward = 0) or tlpc_lpc_backward (backward =
+ * 1) runs for these arguments: the resolved strategy (TLPC_STRATEGY_*, never
+ * AUTO) and the chunk length (0 for the per-signal strategy, whose tiles are
+ * fixed). Host only, launches nothing. No reference counterpart (the reference
+ * has one CUDA kernel per op, torchlpc/csrc/cuda/lpc.cu:115-175).
+ */
+int tlpc_lpc_plan(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags, int32_t strategy,
+                  int32_t chunk_len, int32_t backward, int32_t* strategy_out,
+                  int32_t* chunk_len_out);
 
 /*
  * Size of the leading workspace region that a forward and a backward with these

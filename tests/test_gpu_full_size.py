@@ -67,15 +67,16 @@ def test_time_invariant_b1024_t16384_n32_full_size():
 def test_reflection_b256_t48000_n16_timed_plan():
     """The B=256, T=48000, N=16 step exactly as bench.py times it: the same seeded
     inputs (generated on the GPU), the automatic plan, lpc_keep and then
-    lpc_backward reusing the kept workspace. y, zf, dx, dA and dzi of all 256
-    signals within 1e-5 (relative L2) of the FP64 oracle."""
+    lpc_backward (reusing the kept workspace when the plan keeps one). y, zf, dx,
+    dA and dzi of all 256 signals within 1e-5 (relative L2) of the FP64 oracle."""
     import bench
+    from torchlpc_amd._ext import plan_of
 
     B, T, N = 256, 48000, 16
     x, A, zi, w = bench.make_inputs(B, T, N, torch.device("cuda:0"))
+    print("plan", plan_of(x, A), plan_of(w, A, backward=True), flush=True)
     y, ws = ops().lpc_keep(x, A, zi, 0, 0)
-    assert ws.numel() > 0, "the forward kept no workspace for the backward"
-    dx, dA, dzi = ops().lpc_backward(w, A, zi, y, True, True, True, 0, 0, ws)
+    dx, dA, dzi = ops().lpc_backward(w, A, zi, y, True, True, True, 0, 0, ws if ws.numel() else None)
     torch.cuda.synchronize()
     up = lambda t: host(t).astype(np.float64)  # noqa: E731
     x64, A64, zi64, w64 = up(x), up(A), up(zi), up(w)
@@ -121,6 +122,40 @@ def test_hip_graph_capture_replays(N, T, B):
         outs = step()
     for r in range(4):
         junk = torch.randn(1 << 20, device="cuda") * 2.0  # eager work between replays
+        X.mul_(1.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        for e, o in zip(eager, outs):
+            assert torch.equal(e, o), (r, float((e - o).abs().max()))
+        del junk
+
+
+def test_hip_graph_capture_replays_signal_strategy():
+    """The per-signal tiled kernels (strategy 2, no workspace) captured in one HIP
+    graph, forward and backward: every replay equals the eager result bit for bit."""
+    rng = np.random.default_rng(77)
+    from tests.gpu_util import stable_inputs
+
+    B, T, N = 5, 3000, 16
+    x, A, zi = stable_inputs(rng, B, T, N, np.float32, kmax=0.8, hop=240)
+    g = stable_inputs(rng, B, T, 1, np.float32)[0]
+    X, AA, Z, G = dev(x), dev(A), dev(zi), dev(g)
+
+    def step():
+        y = ops().lpc(X, AA, Z, 2, 0)
+        return (y, *ops().lpc_backward(G, AA, Z, y, True, True, True, 2, 0))
+
+    eager = [t.clone() for t in step()]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = step()
+    for r in range(4):
+        junk = torch.randn(1 << 20, device="cuda") * 2.0
         X.mul_(1.0)
         graph.replay()
         torch.cuda.synchronize()

@@ -380,14 +380,66 @@ def test_order16_unaligned_views(T):
             assert rel(host(o), r) <= 1e-5, (xa, Aa, ga)
 
 
-@pytest.mark.parametrize("T,chunk", [(1, 0), (15, 0), (17, 16), (1000, 16), (3583, 64), (3585, 64), (48000, 0), (20000, 32)])
-def test_signal_strategy_forward(T, chunk):
-    """Per-signal fused kernel (order 16, float32) against the FP64 oracle."""
-    rng = np.random.default_rng(T + chunk)
-    x, A, zi = stable_inputs(rng, 3, T, 16, np.float32, kmax=0.8, hop=240)
-    y = host(ops().lpc(dev(x), dev(A), dev(zi), 2, chunk))
-    ref = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
-    assert rel(y, ref) <= 1e-5, rel(y, ref)
+@pytest.mark.parametrize("B,T", [(3, 1), (3, 15), (2, 16), (3, 17), (3, 1000), (2, 1023), (2, 1024), (3, 1025),
+                                 (3, 3583), (2, 3585), (3, 48000), (67, 2048)])
+def test_signal_strategy_forward_backward(B, T):
+    """Per-signal tiled kernels (order 16, float32, strategy 2): y, dx, dA, dzi
+    against the FP64 oracle, at lengths around the tile (1024) and sub-chunk (64)
+    sizes, with T a multiple of 4 (float4 streams) and not."""
+    rng = np.random.default_rng(B * 100003 + T)
+    x, A, zi = stable_inputs(rng, B, T, 16, np.float32, kmax=0.8, hop=240)
+    g = rng.standard_normal((B, T)).astype(np.float32)
+    y = ops().lpc(dev(x), dev(A), dev(zi), 2, 0)
+    dx, dA, dzi = ops().lpc_backward(dev(g), dev(A), dev(zi), y, True, True, True, 2, 0)
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    rx, rA, rzi = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    errs = [rel(host(y), y64), rel(host(dx), rx), rel(host(dA), rA), rel(host(dzi), rzi)]
+    assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("need", [(True, False, False), (False, True, False), (False, False, True), (True, False, True)])
+def test_signal_strategy_need_flags(need):
+    """The per-signal adjoint writes only the requested gradients; without dA it
+    never reads y (passed as a tensor of NaNs here)."""
+    rng = np.random.default_rng(5)
+    B, T = 3, 2500
+    x, A, zi = stable_inputs(rng, B, T, 16, np.float32, kmax=0.8, hop=240)
+    g = rng.standard_normal((B, T)).astype(np.float32)
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    ref = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    y = dev(y64.astype(np.float32)) if need[1] else torch.full((B, T), float("nan"), device="cuda")
+    outs = ops().lpc_backward(dev(g), dev(A), dev(zi), y, *need, 2, 0)
+    for o, r, n in zip(outs, ref, need):
+        if n:
+            assert rel(host(o), r) <= 1e-5
+        else:
+            assert o.numel() == 0
+
+
+def test_signal_strategy_unaligned_views():
+    """Views that start 4 bytes into their storage take the per-signal kernels'
+    element-wise variant (same results within the bar)."""
+    rng = np.random.default_rng(6)
+    B, T = 2, 3000
+    x, A, zi = stable_inputs(rng, B, T, 16, np.float32, kmax=0.8, hop=240)
+    g = rng.standard_normal((B, T)).astype(np.float32)
+
+    def unaligned(a):
+        t = dev(a)
+        buf = torch.empty(t.numel() + 1, dtype=t.dtype, device=t.device)
+        v = buf[1:].view(t.shape)
+        v.copy_(t)
+        return v
+
+    X, AA, G = unaligned(x), unaligned(A), unaligned(g)
+    assert X.data_ptr() % 16 and AA.data_ptr() % 16 and G.data_ptr() % 16
+    Z = dev(zi)
+    y = ops().lpc(X, AA, Z, 2, 0)
+    dx, dA, dzi = ops().lpc_backward(G, AA, Z, y, True, True, True, 2, 0)
+    y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
+    rx, rA, rzi = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
+    errs = [rel(host(y), y64), rel(host(dx), rx), rel(host(dA), rA), rel(host(dzi), rzi)]
+    assert max(errs) <= 1e-5, errs
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])

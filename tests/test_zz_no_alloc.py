@@ -92,11 +92,12 @@ def test_noalloc_forward_time_invariant():
 
 
 def test_noalloc_forward_rejects_oversize():
-    """Sizes whose workspace exceeds the static scratch return TLPC_ESCRATCH; the
-    query needs no device memory (the pointers are never read)."""
+    """Sizes whose workspace exceeds the static scratch return TLPC_ESCRATCH (here
+    the chunked strategy, whose summaries grow with the batch); the check needs no
+    device memory (the pointers are never read)."""
     lib = _lib()
-    B, T, N = 4096, 48000, 16
-    assert lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, N, 0, 0, 0) > SCRATCH_BYTES
+    B, T, N, chunked = 4096, 48000, 16, 1
+    assert lib.tlpc_lpc_forward_workspace(TLPC_F32, B, T, N, 0, chunked, 0) > SCRATCH_BYTES
     stream = torch.cuda.current_stream().cuda_stream
-    rc = lib.tlpc_lpc_forward_noalloc(TLPC_F32, 1, 1, 1, 1, B, T, N, 0, 0, 0, stream)
+    rc = lib.tlpc_lpc_forward_noalloc(TLPC_F32, 1, 1, 1, 1, B, T, N, 0, chunked, 0, stream)
     assert rc == TLPC_ESCRATCH

@@ -83,6 +83,31 @@ def keep_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int, chunk_
     return max(fwd, bwd)
 
 
+def plan_of(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chunk_len: int = 0,
+            backward: bool = False) -> tuple[int, int]:
+    """(strategy, chunk length) that the forward (or backward) runs for these
+    inputs: the C ABI's host-side plan query ``tlpc_lpc_plan``, which launches
+    nothing. Strategy codes: 1 chunked, 2 per-signal, 3 serial."""
+    import ctypes
+
+    lib = ctypes.CDLL(str(HIP_LIB))
+    i64, i32 = ctypes.c_int64, ctypes.c_int32
+    fn = lib.tlpc_lpc_plan
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32, i32,
+                   ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    dt = _DTYPE_CODE[x.dtype]
+    B, T = x.shape
+    N = A.shape[-1]
+    ti = A.dim() == 2 or (A.shape[1] > 1 and A.stride(1) == 0)
+    flags = (8 if ti else 0) | (7 if backward else 0)
+    s_out, l_out = i32(0), i32(0)
+    rc = fn(dt, B, T, N, flags, strategy, chunk_len, int(backward), ctypes.byref(s_out), ctypes.byref(l_out))
+    if rc != 0:
+        raise RuntimeError(f"tlpc_lpc_plan failed ({rc})")
+    return int(s_out.value), int(l_out.value)
+
+
 def ops():
     """The ``torch.ops.torchlpc_amd`` namespace; raises if the library is missing."""
     if not EXTENSION_LOADED:

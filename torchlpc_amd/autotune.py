@@ -99,8 +99,10 @@ def clear() -> None:
 
 def default_candidates(T: int, N: int) -> Iterable[Tuple[int, int]]:
     """(strategy, chunk_len) pairs worth timing for a signal length and order."""
-    blk = 16 if N < 16 else (16 if N <= 16 else 32)
+    blk = 32 if N > 16 else 16
     out = [(STRATEGY_AUTO, 0)]
+    if N == 16:
+        out.append((STRATEGY_SIGNAL, 0))
     L = max(blk, 64)
     while L <= min(8192, max(T, blk)):
         out.append((STRATEGY_CHUNKED, L))

@@ -120,17 +120,31 @@ inline int pick_chunk_ti(int64_t B, int64_t T, int np, int requested) {
   return (int)((L + blk - 1) / blk * blk);
 }
 
-// The per-signal fused kernel exists for the float32, order-16, time-varying
-// forward; it needs enough signals to occupy the chip (one workgroup each).
-inline bool signal_ok(int dtype, int64_t B, int n, int flags, bool backward, int neg) {
-  return dtype == TLPC_F32 && n == 16 && !(flags & TLPC_TIME_INVARIANT) && !backward && !neg &&
-         B >= 1;
+// The per-signal tiled kernels (one workgroup per signal, every sample of A read
+// once per direction) serve float32, order-16, time-varying filters, forward and
+// backward.
+inline bool signal_ok(int dtype, int n, int flags, int neg) {
+  return dtype == TLPC_F32 && n == 16 && !(flags & TLPC_TIME_INVARIANT) && !neg;
+}
+
+// AUTO takes the per-signal kernels when the signals fill the chip: one
+// workgroup (one signal) per compute unit, so a batch of B signals takes
+// ceil(B / CUs) rounds of the per-signal time. Measured at T = 48000 the chunked
+// kernels take about 1.8 times the per-signal time for a batch of CUs signals
+// (and scale with B), so the per-signal kernels win while
+// ceil(B / CUs) < 1.8 * B / CUs.
+inline bool signal_auto(int64_t B) {
+  const int64_t cus = device_cus();
+  if (B < cus) return false;
+  const int64_t rounds = (B + cus - 1) / cus;
+  return rounds * cus * 10 < 18 * B;
 }
 
 inline int pick_strategy(int strategy, int np, bool sig_ok, int64_t B) {
   if (np == 0) return TLPC_STRATEGY_SERIAL;
   if (strategy == TLPC_STRATEGY_SERIAL) return TLPC_STRATEGY_SERIAL;
   if (strategy == TLPC_STRATEGY_SIGNAL && sig_ok) return TLPC_STRATEGY_SIGNAL;
+  if (strategy == TLPC_STRATEGY_AUTO && sig_ok && signal_auto(B)) return TLPC_STRATEGY_SIGNAL;
   return TLPC_STRATEGY_CHUNKED;
 }
 
@@ -171,11 +185,10 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   Plan p;
   const size_t es = elem_size(dtype);
   p.np = compiled_order(n);
-  p.strategy = pick_strategy(strategy, p.np, signal_ok(dtype, B, n, flags, backward, neg), B);
+  p.strategy = pick_strategy(strategy, p.np, signal_ok(dtype, n, flags, neg), B);
   if (p.strategy == TLPC_STRATEGY_SIGNAL) {
-    p.L = signal_chunk_len(T, chunk_len);
     p.P = 1;
-    return p;  // no workspace
+    return p;  // summaries and carries stay on chip: no workspace
   }
   if (p.strategy == TLPC_STRATEGY_SERIAL) {
     // the serial adjoint keeps lambda in the workspace when dx is not requested
@@ -319,7 +332,10 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
     return backward ? launch_serial_bwd<S>(a, st) : launch_serial_fwd<S>(a, st);
   }
   if (p.strategy == TLPC_STRATEGY_SIGNAL) {
-    if constexpr (sizeof(S) == 4) return launch_signal_fwd16(a, st);
+    if constexpr (sizeof(S) == 4) {
+      a.dA = need_da ? static_cast<S*>(dA) : nullptr;
+      return backward ? launch_tile16_bwd(a, st) : launch_tile16_fwd(a, st);
+    }
     return TLPC_EINVAL;
   }
   using W = typename Wide<S>::type;
@@ -515,6 +531,19 @@ TLPC_EXPORT int tlpc_lpc_backward(int dtype, const void* g, const void* A, const
                                   void* hip_stream) {
   return run_dtype(dtype, g, A, zi, const_cast<void*>(y), dx, dA, dzi, B, T, N, flags, strategy,
                    chunk_len, workspace, workspace_bytes, hip_stream, true);
+}
+
+TLPC_EXPORT int tlpc_lpc_plan(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,
+                              int32_t strategy, int32_t chunk_len, int32_t backward,
+                              int32_t* strategy_out, int32_t* chunk_len_out) {
+  const int rc = check_common(dtype, B, T, N);
+  if (rc != TLPC_OK) return rc;
+  if (!strategy_out || !chunk_len_out) return TLPC_EINVAL;
+  const Plan p = make_plan(dtype, B, T, N, backward ? flags : (flags & TLPC_TIME_INVARIANT),
+                           strategy, chunk_len, backward != 0);
+  *strategy_out = p.strategy;
+  *chunk_len_out = p.L;
+  return TLPC_OK;
 }
 
 TLPC_EXPORT size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32_t N,

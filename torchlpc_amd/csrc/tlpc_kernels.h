@@ -134,9 +134,10 @@ int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st);
 int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st);
 int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st);
 
-// Per-signal fused kernels (order 16, float32): one workgroup per signal.
-int launch_signal_fwd16(ChunkArgs<float> a, hipStream_t st);
-int signal_chunk_len(int64_t T, int requested);
+// Per-signal tiled kernels (order 16, float32, time-varying): one workgroup per
+// signal, no workspace. a.y may be null in the backward when dA is not wanted.
+int launch_tile16_fwd(ChunkArgs<float> a, hipStream_t st);
+int launch_tile16_bwd(ChunkArgs<float> a, hipStream_t st);
 
 // Coefficient as used by the recurrence: A, or -w for the first-order scan.
 template <typename S>
