@@ -19,9 +19,12 @@ timed on this host's cores.
 
 "Dominant kernel" in the roofline object is the dominant op of the step (the
 backward at this shape): its algorithmic bytes per call over its event-timed
-duration. The op is a short fixed sequence of kernels (OP_KERNELS); the rocprof
-trace summary adds their average durations for comparison, and `traffic` is
-their counter-measured HBM bytes per call from the committed summary.
+duration. The op is one kernel (per-signal strategy) or a short fixed sequence
+of kernels (chunked strategy; OP_KERNELS_BY_STRATEGY); the rocprof trace summary
+adds their average durations for comparison, and `traffic` is their
+counter-measured HBM bytes per call from the committed summary. Before timing,
+the project tuner picks the (strategy, chunk length) plan on the same inputs
+(--no-tune: the library heuristic).
 """
 from __future__ import annotations
 
@@ -36,11 +39,17 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 COPY_BYTES = 1 << 30  # the --calibrate-copy kernel reads and writes this many bytes
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
-# kernels of one forward and one backward call at the order-16 float32 shape
-# (name prefixes: template arguments after these select the load variant)
-OP_KERNELS = {
-    "lpc (forward)": ("sum_fwd<true", "carry16<true>", "rerun_fwd"),
-    "lpc_backward (fused adjoint)": ("sum_adj", "carry16<false>", "rerun_adj<true"),
+# kernels of one forward and one backward call at the order-16 float32 shape, per
+# strategy (name prefixes: template arguments after these select the load variant)
+OP_KERNELS_BY_STRATEGY = {
+    2: {  # per-signal tiled kernels
+        "lpc (forward)": ("tile_fwd",),
+        "lpc_backward (fused adjoint)": ("tile_bwd",),
+    },
+    1: {  # chunked scan
+        "lpc (forward)": ("sum_fwd<true", "carry16<true>", "rerun_fwd"),
+        "lpc_backward (fused adjoint)": ("sum_adj", "carry16<false>", "rerun_adj<true"),
+    },
 }
 
 
@@ -425,10 +434,11 @@ def trace_summary(db_path):
         avg[k] = d / 1e3
         if "tlpc" in name or k.startswith("neg"):
             lines.append(f"{k:40s} {n:5d} {d / 1e3:10.1f}")
-    for op, ks in OP_KERNELS.items():
-        parts = [_op_of(avg, k) for k in ks]
-        if all(p is not None for p in parts):
-            lines.append(f"op {op}: sum of kernel averages = {sum(parts):.1f} us ({' + '.join(ks)})")
+    for kernels in OP_KERNELS_BY_STRATEGY.values():
+        for op, ks in kernels.items():
+            parts = [_op_of(avg, k) for k in ks]
+            if all(p is not None for p in parts):
+                lines.append(f"op {op}: sum of kernel averages = {sum(parts):.1f} us ({' + '.join(ks)})")
     return "\n".join(lines)
 
 
@@ -463,23 +473,24 @@ def pmc_summary(fetch_dir, write_dir):
             continue
         t = fetch[k] * 1024 * ff + write[k] * 1024 * fw
         traffic[k] = t
-        if any(k.startswith(p) for ks in OP_KERNELS.values() for p in ks):
+        if any(k.startswith(p) for kernels in OP_KERNELS_BY_STRATEGY.values() for ks in kernels.values() for p in ks):
             lines.append(f"{k:40s} {fetch[k]:12.0f} {write[k]:12.0f} {t / 1e6:11.1f}")
-    for op, ks in OP_KERNELS.items():
-        parts = [_op_of(traffic, k) for k in ks]
-        if all(p is not None for p in parts):
-            lines.append(f"traffic_bytes_per_call {op}: {sum(parts):.0f}")
+    for strategy, kernels in OP_KERNELS_BY_STRATEGY.items():
+        for op, ks in kernels.items():
+            parts = [_op_of(traffic, k) for k in ks]
+            if all(p is not None for p in parts):
+                lines.append(f"traffic_bytes_per_call strategy{strategy} {op}: {sum(parts):.0f}")
     return "\n".join(lines)
 
 
-def committed_traffic(op, B, T, N):
-    """HBM bytes per call of `op` from the committed counter summary, if there is
-    one for this shape."""
+def committed_traffic(op, B, T, N, strategy):
+    """HBM bytes per call of `op` under `strategy` from the committed counter
+    summary, if there is one for this shape."""
     path = os.path.join(PROFILES, f"lpc_traffic_B{B}_T{T}_N{N}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith(f"traffic_bytes_per_call {op}:"):
+        if line.startswith(f"traffic_bytes_per_call strategy{strategy} {op}:"):
             return float(line.rsplit(":", 1)[1])
     return None
 
@@ -496,6 +507,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the library heuristic instead of the tuned (strategy, chunk) choice")
     ap.add_argument("--ragged-order", type=int, default=0,
                     help="with --workload ragged512: run only the signals of this order (one group)")
     ap.add_argument("--workload", default="b256-t48000-n16", choices=["b256-t48000-n16", "b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32"],
@@ -541,8 +554,22 @@ def main():
     x, A, zi, w = make_inputs(B, T, N, dev)
     torch.cuda.synchronize()
     ops = torch.ops.torchlpc_amd
-    s_f, L_f = (args.strategy, args.chunk) if (args.strategy or args.chunk) else autotune.lookup("lpc_forward", x, A)
-    s_b, L_b = (args.strategy, args.chunk) if (args.strategy or args.chunk) else autotune.lookup("lpc_backward", w, A)
+    tuned = None
+    if args.strategy or args.chunk:
+        s_f, L_f = args.strategy, args.chunk
+    elif args.no_tune:
+        s_f, L_f = 0, 0
+    else:
+        # untimed: the project tuner times the candidate plans on these inputs
+        # (the whole step each) and the timed steps below run its choice
+        log("[bench] tuning (strategy, chunk length)")
+        tuned = autotune.tune(x, A, zi, grad=w, reps=3, persist=False)
+        s_f, L_f = tuned["choice"]
+        log(f"[bench] tuned {tuned}")
+    s_b, L_b = s_f, L_f
+    from torchlpc_amd._ext import plan_of
+
+    plan_f, plan_b = plan_of(x, A, s_f, L_f), plan_of(w, A, s_b, L_b, backward=True)
 
     stream = torch.cuda.current_stream()
     n = args.steps
@@ -615,6 +642,9 @@ def main():
             "N": N,
             "strategy_fwd": [s_f, L_f],
             "strategy_bwd": [s_b, L_b],
+            "plan_fwd": list(plan_f),
+            "plan_bwd": list(plan_b),
+            "tuned": tuned,
         },
         "hbm_frac_step": bytes_step / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "fwd_ms": fwd_ms,
@@ -626,9 +656,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": committed_traffic(dom, B, T, N),
+            "traffic": committed_traffic(dom, B, T, N, plan_b[0] if dom.startswith("lpc_backward") else plan_f[0]),
             "algorithmic_bytes": dom_bytes,
-            "kernels": list(OP_KERNELS.get(dom, ())),
+            "kernels": list(OP_KERNELS_BY_STRATEGY.get(plan_b[0] if dom.startswith("lpc_backward") else plan_f[0], {}).get(dom, ())),
         },
     }
 

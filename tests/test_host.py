@@ -113,7 +113,23 @@ def test_autotune_table_roundtrip(tmp_path, monkeypatch):
     autotune.store("lpc_forward", x, A, (1, 256))
     assert autotune.lookup("lpc_forward", x, A) == (1, 256)
     assert (tmp_path / "tune.json").exists()
+    # keyed on the exact length: a choice for T=1000 says nothing about T=1001
+    assert autotune.lookup("lpc_forward", meta(4, 1001), meta(4, 1001, 16)) == (0, 0)
     autotune.clear()
+
+
+def test_autotune_candidates():
+    """Candidates: the heuristic, the per-signal kernels for float32 order 16, and
+    chunked lengths that are multiples of the staging block (16, or 32 above
+    order 16), within the signal."""
+    from torchlpc_amd import autotune
+
+    c16 = autotune.default_candidates(256, 48000, 16, torch.float32)
+    assert c16[0] == (0, 0) and (2, 0) in c16
+    assert all(L % 16 == 0 and 0 < L <= 48000 for s, L in c16 if s == 1) and len(c16) >= 5
+    c32 = autotune.default_candidates(8, 5000, 32, torch.float64)
+    assert (2, 0) not in c32 and all(L % 32 == 0 for s, L in c32 if s == 1)
+    assert autotune.default_candidates(3, 0, 4, torch.float32) == [(0, 0)]
 
 
 def test_meta_order16_keeps_forward_summaries():

@@ -1,14 +1,23 @@
 """Project-written autotuner for the filter launches.
 
-The kernels take two launch parameters: the strategy (chunked multi-kernel scan,
-one workgroup per signal, or serial) and the chunk length (samples per
-independently scanned chunk). ``tune()`` times the candidates on the caller's
-real tensors with HIP events and stores the winner per key
-(op, dtype, device, B, log2-bucket of T, N, time-invariant); ``lookup()`` returns
-the stored choice or ``(0, 0)`` (library heuristic). The table is kept in memory
-and persisted as JSON (``TORCHLPC_AMD_TUNE_CACHE``, default
+The kernels take two launch parameters:
+
+* the strategy: the per-signal tiled kernels (one workgroup, 8 waves, per
+  signal; float32 order 16), the chunked multi-kernel scan, or serial;
+* the chunk length of the chunked scan (samples per independently scanned
+  chunk). It sets the parallelism inside a signal: the order-16 reruns run one
+  4-lane quad per chunk, i.e. one wave per 16 chunks, so a signal of T samples
+  gets ceil(T / chunk / 16) rerun waves, and the carry chain one link per chunk.
+
+``tune()`` times candidate (strategy, chunk length) pairs on the caller's real
+tensors with HIP events, as the whole step a training call runs (forward, then
+the fused backward with the forward's plan and kept workspace), and stores the
+fastest per key (op, dtype, device, B, T, N, time-invariant); ``lookup()``
+returns the stored choice or ``(0, 0)`` (the library heuristic). The table is
+kept in memory and persisted as JSON (``TORCHLPC_AMD_TUNE_CACHE``, default
 ``~/.cache/torchlpc_amd/tune.json``). Lookups are read-mostly under a lock and
-never synchronise the device, so they are safe inside hipGraph capture.
+never synchronise the device, so they are safe inside hipGraph capture;
+``tune()`` synchronises and must run outside capture.
 
 The reference has no tuner: its chunking constants are hard-coded
 (``torchlpc/csrc/cuda/linear_recurrence.cu:238-239``, ``torchlpc/recurrence.py:49,64``).
@@ -16,11 +25,10 @@ The reference has no tuner: its chunking constants are hard-coded
 from __future__ import annotations
 
 import json
-import math
 import os
 import threading
 from pathlib import Path
-from typing import Dict, Iterable, Optional, Tuple
+from typing import Dict, Iterable, List, Optional, Tuple
 
 import torch
 
@@ -43,9 +51,8 @@ def _key(op: str, x: torch.Tensor, A: torch.Tensor) -> str:
     B, T = x.shape[0], x.shape[1]
     N = A.shape[-1]
     ti = A.dim() == 2 or (A.dim() == 3 and A.shape[1] > 1 and A.stride(1) == 0)
-    tb = int(math.log2(T)) if T > 0 else -1
     dev = torch.cuda.get_device_name(x.device) if x.is_cuda else "cpu"
-    return f"{op}|{x.dtype}|{dev}|B{B}|T{tb}|N{N}|ti{int(ti)}"
+    return f"{op}|{x.dtype}|{dev}|B{B}|T{T}|N{N}|ti{int(ti)}"
 
 
 def _load() -> None:
@@ -97,30 +104,60 @@ def clear() -> None:
         _loaded = True
 
 
-def default_candidates(T: int, N: int) -> Iterable[Tuple[int, int]]:
-    """(strategy, chunk_len) pairs worth timing for a signal length and order."""
+def default_candidates(B: int, T: int, N: int, dtype: torch.dtype) -> List[Tuple[int, int]]:
+    """(strategy, chunk_len) pairs worth timing for a batch shape.
+
+    The heuristic (0, 0) always; the per-signal kernels for float32 order 16;
+    the chunked scan at chunk lengths that give between 1/4 and 4 times as many
+    rerun lanes as compute units can hold (chunk lengths are multiples of 16,
+    32 from order 17 up)."""
     blk = 32 if N > 16 else 16
-    out = [(STRATEGY_AUTO, 0)]
-    if N == 16:
+    out: List[Tuple[int, int]] = [(STRATEGY_AUTO, 0)]
+    if N == 16 and dtype == torch.float32:
         out.append((STRATEGY_SIGNAL, 0))
-    L = max(blk, 64)
-    while L <= min(8192, max(T, blk)):
-        out.append((STRATEGY_CHUNKED, L))
-        L *= 2
+    if T <= 0:
+        return out
+    for chunks_per_signal in (16, 32, 64, 128, 256):
+        L = -(-T // chunks_per_signal)
+        L = max(blk, -(-L // blk) * blk)
+        if L <= T and (STRATEGY_CHUNKED, L) not in out:
+            out.append((STRATEGY_CHUNKED, L))
     return out
 
 
 def _time(fn, reps: int) -> float:
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
+    """Median per-call milliseconds of `fn` over `reps` event-timed calls."""
     fn()
     torch.cuda.synchronize()
-    start.record()
+    times = []
     for _ in range(reps):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
         fn()
-    end.record()
-    torch.cuda.synchronize()
-    return start.elapsed_time(end) / reps
+        end.record()
+        torch.cuda.synchronize()
+        times.append(start.elapsed_time(end))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def step_fn(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor, grad: Optional[torch.Tensor],
+            choice: Tuple[int, int]):
+    """The call sequence one training step runs for a (strategy, chunk) choice:
+    the forward keeping its workspace, then (with `grad`) the fused backward with
+    the same plan, reusing the kept summaries when there are any."""
+    from ._ext import ops
+
+    s, L = choice
+
+    def run():
+        if grad is None:
+            return ops().lpc(x, A, zi, s, L)
+        y, ws = ops().lpc_keep(x, A, zi, s, L)
+        return y, ops().lpc_backward(grad, A, zi, y, True, True, True, s, L, ws if ws.numel() else None)
+
+    return run
 
 
 def tune(
@@ -131,27 +168,27 @@ def tune(
     candidates: Optional[Iterable[Tuple[int, int]]] = None,
     reps: int = 5,
     persist: bool = True,
-) -> Dict[str, Tuple[int, int]]:
-    """Time forward (and, with ``grad``, the fused backward) over the candidates.
+) -> Dict[str, object]:
+    """Time the step (forward, and with ``grad`` the fused backward) for every
+    candidate and store the fastest for ``lpc_forward`` and ``lpc_backward``.
 
-    Must not be called during hipGraph capture (it synchronises).
+    Returns ``{"choice": (s, L), "ms": {"s,L": median ms, ...},
+    "heuristic_ms": ms of (0, 0)}``. Must not be called during hipGraph capture
+    (it synchronises).
     """
-    from ._ext import ops
-
     assert x.is_cuda, "tuning needs GPU tensors"
+    B, T = x.shape
     N = A.shape[-1]
-    cands = list(candidates) if candidates is not None else list(default_candidates(x.shape[1], N))
-    best_f = min(cands, key=lambda c: _time(lambda: ops().lpc(x, A, zi, c[0], c[1]), reps))
-    store("lpc_forward", x, A, best_f, persist)
-    result = {"lpc_forward": best_f}
+    cands = list(candidates) if candidates is not None else default_candidates(B, T, N, x.dtype)
+    if (STRATEGY_AUTO, 0) not in cands:
+        cands.insert(0, (STRATEGY_AUTO, 0))
+    ms = {c: _time(step_fn(x, A, zi, grad, c), reps) for c in cands}
+    best = min(cands, key=lambda c: ms[c])
+    store("lpc_forward", x, A, best, persist)
     if grad is not None:
-        y = ops().lpc(x, A, zi, best_f[0], best_f[1])
-        best_b = min(
-            cands,
-            key=lambda c: _time(
-                lambda: ops().lpc_backward(grad, A, zi, y, True, True, True, c[0], c[1]), reps
-            ),
-        )
-        store("lpc_backward", grad, A, best_b, persist)
-        result["lpc_backward"] = best_b
-    return result
+        store("lpc_backward", grad, A, best, persist)
+    return {
+        "choice": best,
+        "ms": {f"{c[0]},{c[1]}": v for c, v in ms.items()},
+        "heuristic_ms": ms[(STRATEGY_AUTO, 0)],
+    }
