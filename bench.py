@@ -296,8 +296,8 @@ def workload_inputs(name, device, seed):
         A = torch.stack([-2 * r * torch.cos(th), (r * r).expand(B, T)], -1)
         return (torch.randn(B, T, generator=g, **f64), A, torch.randn(B, 2, generator=g, **f64),
                 torch.randn(B, T, generator=g, **f64))
-    if name == "ti-b1024-t16384-n32":  # B=1024, T=16384, N=32, time-invariant a[B, N]
-        B, T, N = 1024, 16384, 32
+    if name in ("ti-b1024-t16384-n32", "ti-b4-t1048576-n32"):  # time-invariant a[B, N], N=32
+        B, T, N = (1024, 16384, 32) if name == "ti-b1024-t16384-n32" else (4, 1 << 20, 32)
         a = _step_up((torch.rand(B, N, generator=g, **f64) * 2 - 1) * 0.8)
         return (torch.randn(B, T, generator=g, **f64), a, torch.randn(B, N, generator=g, **f64) * 0.1,
                 torch.randn(B, T, generator=g, **f64))
@@ -327,12 +327,12 @@ def bench_workload(name, steps, warmup, ragged_order=0):
         dx, dA, dzi = torch.empty_like(x), torch.empty_like(A), torch.empty_like(zi)
 
         def fwd(grp):
-            n, table, Kg, G, maxp, L = grp
-            ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L)
+            n, table, Kg, G, maxp, L, ext = grp
+            ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L, ext)
 
         def bwd(grp):
-            n, table, Kg, G, maxp, L = grp
-            ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True)
+            n, table, Kg, G, maxp, L, ext = grp
+            ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True, ext)
 
         if ragged_order and ragged_order not in orders:
             raise SystemExit(f"--ragged-order {ragged_order}: no signal of that order (orders are 2, 4, 8, 16, 32)")
@@ -365,7 +365,7 @@ def bench_workload(name, steps, warmup, ragged_order=0):
             step()
         run = graph.replay
     else:
-        x64, A64, zi64, w64 = workload_inputs(name, dev, {"b16-t8192-n4": 1001, "b4-t1048576-n2": 1003, "ti-b1024-t16384-n32": 1005}[name])
+        x64, A64, zi64, w64 = workload_inputs(name, dev, {"b16-t8192-n4": 1001, "b4-t1048576-n2": 1003, "ti-b1024-t16384-n32": 1005, "ti-b4-t1048576-n32": 1006}[name])
         x, A, zi = x64.to(f32), A64.to(f32).contiguous(), zi64.to(f32)
         w = w64.to(f32) if w64 is not None else None
         B, T = x.shape
@@ -511,7 +511,7 @@ def main():
                     help="time the library heuristic instead of the tuned (strategy, chunk) choice")
     ap.add_argument("--ragged-order", type=int, default=0,
                     help="with --workload ragged512: run only the signals of this order (one group)")
-    ap.add_argument("--workload", default="b256-t48000-n16", choices=["b256-t48000-n16", "b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32"],
+    ap.add_argument("--workload", default="b256-t48000-n16", choices=["b256-t48000-n16", "b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32", "ti-b4-t1048576-n32"],
                     help="the other workloads print their own line (the default is B=256, T=48000, N=16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one copy-like kernel of known size (for counter scaling)")

@@ -64,6 +64,31 @@ def test_time_invariant_b1024_t16384_n32_full_size():
     assert max(errs) <= 1e-5, errs
 
 
+def test_time_invariant_long_signals_b2_t1m_n32():
+    """Two time-invariant order-32 signals of 2^20 samples (about 1024 chunks of
+    ~1264 samples each, so a 1024-link carry chain per signal): y, dx, da, dzi
+    within 1e-5 of the FP64 oracle."""
+    from torchlpc_amd._ext import plan_of
+
+    x, a, zi, w = (v.astype(np.float32) for v in O.make_ti_b1024_t16384_n32(B=2, T=1 << 20, seed=1006))
+    B, T = x.shape
+    N = a.shape[1]
+    X, AA = dev(x), dev(a)
+    s, L = plan_of(X, AA)
+    print("plan", (s, L), flush=True)
+    assert s == 1 and 1024 <= L <= 2048
+    yd = ops().lpc(X, AA, dev(zi))
+    dx, da, dzi = ops().lpc_backward(dev(w), AA, dev(zi), yd, True, True, True)
+    up = lambda v: v.astype(np.float64)  # noqa: E731
+    A3 = np.broadcast_to(up(a)[:, None, :], (B, T, N))
+    y64 = O.lpc(up(x), A3, up(zi))
+    rx, rA, rzi = O.lpc_backward(up(w), A3, up(zi), y64)
+    del A3
+    errs = [rel(host(yd), y64), rel(host(dx), rx), rel(host(da), rA.sum(1)), rel(host(dzi), rzi)]
+    print("ti-b2-t1m-n32 rel errors", errs, flush=True)
+    assert max(errs) <= 1e-5, errs
+
+
 def test_reflection_b256_t48000_n16_timed_plan():
     """The B=256, T=48000, N=16 step exactly as bench.py times it: the same seeded
     inputs (generated on the GPU), the automatic plan, lpc_keep and then

@@ -280,18 +280,47 @@ def test_order16_large_batch():
             assert rel(host(o), r) <= 1e-5
 
 
-def test_autograd_order16_uses_kept_summaries():
+def test_autograd_order16_uses_kept_summaries(monkeypatch):
     """LPC.apply on float32 order-16 inputs that require grad keeps the forward's
-    workspace for the backward; the gradients match the FP64 oracle."""
+    workspace for the backward (the backward is handed it: checked by wrapping the
+    op namespace), the workspace goes through saved-tensor hooks, and the
+    gradients match the FP64 oracle. Under no_grad nothing is kept."""
     import torchlpc_amd
+    from torchlpc_amd import core
 
+    real = core.ops()
+    calls = []
+
+    class Spy:
+        def __getattr__(self, name):
+            return getattr(real, name)
+
+        def lpc_keep(self, *a):
+            calls.append(("keep",))
+            return real.lpc_keep(*a)
+
+        def lpc_backward(self, *a):
+            calls.append(("backward", a[-1] if len(a) == 10 else None))
+            return real.lpc_backward(*a)
+
+    monkeypatch.setattr(core, "ops", lambda: Spy())
     rng = np.random.default_rng(77)
     x, A, zi = stable_inputs(rng, 2, 3000, 16, np.float32, kmax=0.8, hop=240)
     g = stable_inputs(rng, 2, 3000, 1, np.float32)[0]
     X, AA, Z = (dev(v).requires_grad_() for v in (x, A, zi))
-    y = torchlpc_amd.LPC.apply(X, AA, Z)
+    packed = []
+    with torch.autograd.graph.saved_tensors_hooks(lambda t: packed.append(t.dtype) or t, lambda t: t):
+        y = torchlpc_amd.LPC.apply(X, AA, Z)
     assert y.grad_fn is not None
+    assert torch.uint8 in packed, "the kept workspace is not a saved tensor"
     y.backward(dev(g))
+    assert calls[0] == ("keep",)
+    summary = calls[-1][1]
+    assert calls[-1][0] == "backward" and summary is not None and summary.numel() > 0
+    calls.clear()
+    with torch.no_grad():
+        torchlpc_amd.LPC.apply(X, AA, Z)
+    assert ("keep",) not in calls
     y64 = O.lpc(up(x, np.float32), up(A, np.float32), up(zi, np.float32))
     rx, rA, rzi = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
     assert rel(host(y), y64) <= 1e-5

@@ -97,8 +97,8 @@ def test_ragged_full_workload_and_graph_replay():
         dx, da, dzi = torch.empty_like(X), torch.empty_like(AA), torch.empty_like(Z)
 
         def bwd(grp):
-            n, table, K, G, maxp, L = grp
-            ops.lpc_ragged_backward_(Gd, AA, Z, y, table, dx, da, dzi, K, G, maxp, n, L, True, True, True)
+            n, table, K, G, maxp, L, ext = grp
+            ops.lpc_ragged_backward_(Gd, AA, Z, y, table, dx, da, dzi, K, G, maxp, n, L, True, True, True, ext)
 
         plan.run(bwd)  # order groups on the plan's side streams
         return y, dx, da, dzi
@@ -121,3 +121,34 @@ def test_ragged_full_workload_and_graph_replay():
         torch.cuda.synchronize()
         for e, o in zip(eager, outs):
             assert torch.equal(e, o), r
+
+
+def test_ragged_ops_refuse_tensors_smaller_than_the_plan():
+    """The raw ops check the packed tensors against the extents of the plan's
+    signals on the host, before anything is launched."""
+    from torchlpc_amd import RaggedPlan
+
+    lengths, orders = [300, 1000, 40], [4, 16, 4]
+    dev0 = torch.device("cuda:0")
+    plan = RaggedPlan(lengths, orders, dev0, torch.float32)
+    x = torch.zeros(plan.total_samples, device=dev0)
+    a = torch.zeros(plan.total_coeffs, device=dev0)
+    zi = torch.zeros(plan.total_state, device=dev0)
+    ops = torch.ops.torchlpc_amd
+    n, table, K, G, maxp, L, ext = plan.groups[1]  # order 16: signal 1 ends at sample 1300
+    y = torch.empty_like(x)
+    ops.lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L, ext)  # whole tensors pass
+    short = x[:1299]
+    with pytest.raises(RuntimeError, match="smaller than the plan"):
+        ops.lpc_ragged_(short, a, zi, table, torch.empty_like(short), K, G, maxp, n, L, ext)
+    with pytest.raises(RuntimeError, match="smaller than the plan"):
+        ops.lpc_ragged_(x, a[: ext[1] - 1], zi, table, y, K, G, maxp, n, L, ext)
+    g = torch.zeros_like(x)
+    dx, da, dzi = torch.empty_like(x), torch.empty_like(a), torch.empty_like(zi)
+    with pytest.raises(RuntimeError, match="smaller than the plan"):
+        zs = zi[: ext[2] - 1]
+        ops.lpc_ragged_backward_(g, a, zs, y, table, dx, da, torch.empty_like(zs), K, G, maxp, n, L,
+                                 True, True, True, ext)
+    torch.cuda.synchronize()
+    assert torch.equal(y[300:1300], torch.zeros(1000, device=dev0))  # zero input, zero state
+    del dx, da, dzi

@@ -153,6 +153,64 @@ def test_meta_order16_keeps_forward_summaries():
     assert _ext.keep_workspace_bytes(x, meta(2, 16), 0, 0) == 0
 
 
+def test_plan_query():
+    """tlpc_lpc_plan: the per-signal kernels for a full float32 order-16 batch, the
+    chunked scan elsewhere; time-invariant chunks of about sqrt(1.5 T) samples on
+    long signals (a 1024-link chain, not 256 chunks of 4096 serial steps)."""
+    from torchlpc_amd._ext import plan_of
+
+    assert plan_of(meta(256, 48000), meta(256, 48000, 16)) == (2, 0)
+    s, L = plan_of(meta(256, 48000, dtype=torch.float64), meta(256, 48000, 16, dtype=torch.float64))
+    assert s == 1 and L > 0
+    for N in (2, 32):
+        s, L = plan_of(meta(4, 1 << 20), meta(4, N))
+        assert s == 1 and 1254 <= L <= 1280 + 32, (N, L)
+    assert plan_of(meta(1024, 16384), meta(1024, 32)) == (1, 352)
+    assert plan_of(meta(4, 3000), meta(4, 3000, 16), strategy=1, chunk_len=100)[1] % 16 == 0
+
+
+def test_keep_decision_follows_the_callers_grad_mode(monkeypatch):
+    """LPC.apply keeps the forward's workspace only when a backward can follow: the
+    caller's grad mode on and an input that requires grad. The workspace is then a
+    saved tensor (saved-tensor hooks see it) and reaches lpc_backward."""
+    from torchlpc_amd import core
+
+    real = core.ops()
+    calls = []
+
+    class Spy:
+        def __getattr__(self, name):
+            return getattr(real, name)
+
+        def lpc(self, *a):
+            calls.append("lpc")
+            return real.lpc(*a)
+
+        def lpc_keep(self, *a):
+            calls.append("keep")
+            return real.lpc_keep(*a)
+
+        def lpc_backward(self, *a):
+            calls.append(("backward", a[-1] is not None and a[-1].dtype == torch.uint8))
+            return real.lpc_backward(*a)
+
+    monkeypatch.setattr(core, "ops", lambda: Spy())
+    x, A, zi = meta(2, 5000).requires_grad_(), meta(2, 5000, 16).requires_grad_(), meta(2, 16)
+    packed = []
+    with torch.autograd.graph.saved_tensors_hooks(lambda t: packed.append(t.dtype) or t, lambda t: t):
+        y = LPC.apply(x, A, zi)
+    assert calls == ["keep"] and torch.uint8 in packed
+    torch.autograd.grad(y.sum(), (x, A))
+    assert calls[-1] == ("backward", True)
+    calls.clear()
+    with torch.no_grad():
+        y = LPC.apply(x, A, zi)
+    assert calls == ["lpc"] and y.grad_fn is None
+    calls.clear()
+    LPC.apply(x.detach(), A.detach(), zi)
+    assert calls == ["lpc"]
+
+
 def test_ragged_plan_table_layout():
     """Host plan of a ragged batch: per order, offsets / lengths / first chunk ids,
     then the signal of every chunk (the layout documented in the C header)."""
@@ -161,8 +219,9 @@ def test_ragged_plan_table_layout():
     lengths, orders = [100, 37, 5000, 16, 1], [2, 16, 2, 32, 2]
     p = RaggedPlan(lengths, orders, "cpu")
     assert (p.total_samples, p.total_coeffs, p.total_state) == (5154, 11306, 54)
-    n, table, K, G, maxp, L = p.groups[0]  # order 2: signals 0, 2, 4
+    n, table, K, G, maxp, L, ext = p.groups[0]  # order 2: signals 0, 2, 4
     assert (n, K) == (2, 3) and L % 16 == 0
+    assert ext == [5154, 11306, 54]  # ends of signal 4 in packed x, A and zi
     P = [-(-T // L) for T in (100, 5000, 1)]
     assert G == sum(P) and maxp == max(P)
     t = table.tolist()
@@ -173,3 +232,10 @@ def test_ragged_plan_table_layout():
     assert t[12:16] == [0, P[0], P[0] + P[1], G]
     assert t[16:] == [0] * P[0] + [1] * P[1] + [2] * P[2]
     assert [g[0] for g in p.groups] == [2, 16, 32]
+    assert p.groups[1][6] == [100 + 37, 200 + 37 * 16, 2 + 16]  # order 16: signal 1 alone
+    # a tensor of another dtype than the plan's is refused before any launch
+    from torchlpc_amd.ragged import RaggedLPC
+
+    f64 = torch.float64
+    with pytest.raises(ValueError, match="plan was made for"):
+        RaggedLPC.apply(torch.zeros(5154, dtype=f64), torch.zeros(11306, dtype=f64), torch.zeros(54, dtype=f64), p)

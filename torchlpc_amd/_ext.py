@@ -150,11 +150,11 @@ def _register_fake() -> None:
         return dx, dA, dzi
 
     @torch.library.register_fake(lib + "lpc_ragged_")
-    def _lpc_ragged_fake(x, A, zi, table, y, K, G, max_chunks, N, chunk_len):
+    def _lpc_ragged_fake(x, A, zi, table, y, K, G, max_chunks, N, chunk_len, extents):
         return None
 
     @torch.library.register_fake(lib + "lpc_ragged_backward_")
-    def _lpc_ragged_bwd_fake(g, A, zi, y, table, dx, dA, dzi, K, G, max_chunks, N, chunk_len, nx, na, nz):
+    def _lpc_ragged_bwd_fake(g, A, zi, y, table, dx, dA, dzi, K, G, max_chunks, N, chunk_len, nx, na, nz, extents):
         return None
 
     @torch.library.register_fake(lib + "scan")

@@ -77,18 +77,31 @@ def composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi):
     return dx, dA, dzi
 
 
-# forward -> setup_context hand-over of the kept workspace (same thread, back to back)
+# forward -> setup_context hand-over of the kept workspace (same thread, back to
+# back: Function.apply runs setup_context right after forward)
 _kept = threading.local()
 
 
 def _keep_summaries(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> bool:
-    if torch.compiler.is_compiling():
+    """Whether the forward keeps its workspace for the backward: only when a
+    backward can follow (the caller's grad mode was on and an input requires
+    grad) and not while torch.compile traces (the fake kernels then see the
+    plain op). Function.forward itself runs with grad mode off, so LPC.apply
+    records the caller's mode first."""
+    caller_grad = getattr(_kept, "grad_mode", False)
+    _kept.grad_mode = False
+    if torch.compiler.is_compiling() or not caller_grad:
         return False
     return x.requires_grad or A.requires_grad or zi.requires_grad
 
 
 class LPC(Function):
     """y[t] = x[t] - sum_{i=1..N} A[t,i] y[t-i], with y[-k] = zi[k-1]."""
+
+    @classmethod
+    def apply(cls, *args, **kwargs):
+        _kept.grad_mode = torch.is_grad_enabled()
+        return Function.apply.__func__(cls, *args, **kwargs)
 
     @staticmethod
     def forward(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> torch.Tensor:
@@ -104,25 +117,30 @@ class LPC(Function):
     @staticmethod
     def setup_context(ctx: Any, inputs: List[Any], output: Any) -> Any:
         _, A, zi = inputs
-        ctx.save_for_backward(A, zi, output)
-        ctx.save_for_forward(A, zi, output)
         kept = getattr(_kept, "item", None)
         _kept.item = None
-        ctx.kept = None
+        ctx.kept_plan = None
         if kept is not None and kept[0].data_ptr() == output.data_ptr():
-            ctx.kept = kept[1:]
+            # the workspace is a saved tensor like the others, so saved-tensor
+            # hooks (offloading, checkpointing) see it
+            ctx.kept_plan = kept[2:]
+            ctx.save_for_backward(A, zi, output, kept[1])
+        else:
+            ctx.save_for_backward(A, zi, output)
+        ctx.save_for_forward(A, zi, output)
 
     @staticmethod
     def backward(
         ctx: Any, grad_y: torch.Tensor
     ) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor], Optional[torch.Tensor]]:
-        A, zi, y = ctx.saved_tensors
+        A, zi, y = ctx.saved_tensors[:3]
         need_x, need_A, need_zi = ctx.needs_input_grad[:3]
         if torch.is_grad_enabled():
             return composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi)
-        kept = getattr(ctx, "kept", None)
-        if kept is not None:
-            ws, strategy, chunk = kept
+        kept_plan = getattr(ctx, "kept_plan", None)
+        if kept_plan is not None:
+            ws = ctx.saved_tensors[3]
+            strategy, chunk = kept_plan
             dx, dA, dzi = ops().lpc_backward(
                 grad_y, A, zi, y, need_x, need_A, need_zi, strategy, chunk, ws
             )
@@ -140,7 +158,7 @@ class LPC(Function):
         grad_A: Optional[torch.Tensor],
         grad_zi: Optional[torch.Tensor],
     ) -> torch.Tensor:
-        A, zi, y = ctx.saved_tensors
+        A, zi, y = ctx.saved_tensors[:3]
         N = A.shape[-1]
         tx = grad_x if grad_x is not None else torch.zeros_like(y)
         tzi = grad_zi if grad_zi is not None else torch.zeros_like(zi)

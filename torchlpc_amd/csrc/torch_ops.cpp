@@ -223,7 +223,10 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> scan_backward_hip(
 // ---------------------------------------------------------------------------
 // Ragged batches of one order (offsets and lengths in a device table built by
 // the host plan below).
-std::tuple<at::Tensor, int64_t, int64_t, int64_t> lpc_ragged_plan_cpu(
+// Returns the device-ready table and (G, max chunks, chunk length, x extent,
+// A extent, zi extent): the extents are the end of the furthest signal in the
+// packed x / y, A / dA and zi / dzi tensors, which the launches check.
+std::tuple<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t> lpc_ragged_plan_cpu(
     const at::Tensor& x_off, const at::Tensor& a_off, const at::Tensor& z_off,
     const at::Tensor& lengths, int64_t N, at::ScalarType dtype, int64_t chunk_len) {
   for (const auto* t : {&x_off, &a_off, &z_off, &lengths})
@@ -246,11 +249,24 @@ std::tuple<at::Tensor, int64_t, int64_t, int64_t> lpc_ragged_plan_cpu(
                                 (int32_t)chunk_len, table.data_ptr<int64_t>(), &tlen, &G, &maxP,
                                 &L),
            "ragged plan");
-  return {table, G, maxP, (int64_t)L};
+  int64_t xe = 0, ae = 0, ze = 0;
+  const int64_t* xp = xo.data_ptr<int64_t>();
+  const int64_t* ap = ao.data_ptr<int64_t>();
+  const int64_t* zp = zo.data_ptr<int64_t>();
+  const int64_t* lp = ln.data_ptr<int64_t>();
+  for (int64_t k = 0; k < K; ++k) {
+    TORCH_CHECK(xp[k] >= 0 && ap[k] >= 0 && zp[k] >= 0,
+                "torchlpc_amd::lpc_ragged_plan: offsets must be non-negative");
+    xe = std::max(xe, xp[k] + lp[k]);
+    ae = std::max(ae, ap[k] + lp[k] * N);
+    ze = std::max(ze, zp[k] + N);
+  }
+  return {table, G, maxP, (int64_t)L, xe, ae, ze};
 }
 
 void check_ragged(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
-                  const at::Tensor& table, const char* op) {
+                  const at::Tensor& table, int64_t K, int64_t G, at::IntArrayRef extents,
+                  const char* op) {
   TORCH_CHECK(x.is_cuda(), "torchlpc_amd::", op,
               ": tensors must be on a HIP (MI355X) device; CPU execution is not provided");
   TORCH_CHECK(x.dim() == 1 && A.dim() == 1 && zi.dim() == 1, "torchlpc_amd::", op,
@@ -263,15 +279,21 @@ void check_ragged(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi
               "torchlpc_amd::", op, ": tensors must be on one device");
   TORCH_CHECK(table.scalar_type() == at::kLong && table.is_contiguous(), "torchlpc_amd::", op,
               ": the plan table must be a contiguous int64 device tensor");
+  TORCH_CHECK(table.numel() == 5 * K + 1 + G, "torchlpc_amd::", op, ": table size mismatch");
+  TORCH_CHECK(extents.size() == 3, "torchlpc_amd::", op,
+              ": extents must be (x, A, zi) as returned by lpc_ragged_plan");
+  TORCH_CHECK(x.numel() >= extents[0] && A.numel() >= extents[1] && zi.numel() >= extents[2],
+              "torchlpc_amd::", op, ": packed tensors (x ", x.numel(), ", A ", A.numel(), ", zi ",
+              zi.numel(), " elements) are smaller than the plan's signals need (", extents[0], ", ",
+              extents[1], ", ", extents[2], ")");
 }
 
 void lpc_ragged_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
                     const at::Tensor& table, at::Tensor& y, int64_t K, int64_t G, int64_t maxP,
-                    int64_t N, int64_t chunk_len) {
-  check_ragged(x, A, zi, table, "lpc_ragged_");
+                    int64_t N, int64_t chunk_len, at::IntArrayRef extents) {
+  check_ragged(x, A, zi, table, K, G, extents, "lpc_ragged_");
   TORCH_CHECK(y.sizes() == x.sizes() && y.scalar_type() == x.scalar_type() && y.is_contiguous(),
               "torchlpc_amd::lpc_ragged_: y must be a contiguous tensor like x");
-  TORCH_CHECK(table.numel() == 5 * K + 1 + G, "torchlpc_amd::lpc_ragged_: table size mismatch");
   c10::DeviceGuard guard(x.device());
   const int dt = dtype_code(x);
   const size_t wsb = tlpc_lpc_ragged_workspace(dt, G, (int32_t)N);
@@ -287,12 +309,10 @@ void lpc_ragged_backward_hip(const at::Tensor& g, const at::Tensor& A, const at:
                              const at::Tensor& y, const at::Tensor& table, at::Tensor& dx,
                              at::Tensor& dA, at::Tensor& dzi, int64_t K, int64_t G, int64_t maxP,
                              int64_t N, int64_t chunk_len, bool need_x, bool need_a,
-                             bool need_zi) {
-  check_ragged(g, A, zi, table, "lpc_ragged_backward_");
+                             bool need_zi, at::IntArrayRef extents) {
+  check_ragged(g, A, zi, table, K, G, extents, "lpc_ragged_backward_");
   TORCH_CHECK(y.sizes() == g.sizes() && y.is_contiguous(),
               "torchlpc_amd::lpc_ragged_backward_: y must be a contiguous tensor like g");
-  TORCH_CHECK(table.numel() == 5 * K + 1 + G,
-              "torchlpc_amd::lpc_ragged_backward_: table size mismatch");
   if (need_x) TORCH_CHECK(dx.sizes() == g.sizes() && dx.is_contiguous(), "lpc_ragged_backward_: dx");
   if (need_a) TORCH_CHECK(dA.sizes() == A.sizes() && dA.is_contiguous(), "lpc_ragged_backward_: dA");
   if (need_zi) TORCH_CHECK(dzi.sizes() == zi.sizes() && dzi.is_contiguous(), "lpc_ragged_backward_: dzi");
@@ -332,13 +352,13 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_cpu(
   cpu_unsupported("lpc_backward");
 }
 void lpc_ragged_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, const at::Tensor&,
-                    at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t) {
+                    at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t, at::IntArrayRef) {
   cpu_unsupported("lpc_ragged_");
 }
 void lpc_ragged_backward_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&,
                              const at::Tensor&, const at::Tensor&, at::Tensor&, at::Tensor&,
                              at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t, bool, bool,
-                             bool) {
+                             bool, at::IntArrayRef) {
   cpu_unsupported("lpc_ragged_backward_");
 }
 at::Tensor scan_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t) {
@@ -365,14 +385,14 @@ TORCH_LIBRARY(torchlpc_amd, m) {
       "(Tensor, Tensor, Tensor)");
   m.def(
       "lpc_ragged_plan(Tensor x_off, Tensor a_off, Tensor z_off, Tensor lengths, int N, "
-      "ScalarType dtype, int chunk_len=0) -> (Tensor, int, int, int)");
+      "ScalarType dtype, int chunk_len=0) -> (Tensor, int, int, int, int, int, int)");
   m.def(
       "lpc_ragged_(Tensor x, Tensor A, Tensor zi, Tensor table, Tensor(a!) y, int K, int G, "
-      "int max_chunks, int N, int chunk_len) -> ()");
+      "int max_chunks, int N, int chunk_len, int[] extents) -> ()");
   m.def(
       "lpc_ragged_backward_(Tensor g, Tensor A, Tensor zi, Tensor y, Tensor table, Tensor(a!) dx, "
       "Tensor(b!) dA, Tensor(c!) dzi, int K, int G, int max_chunks, int N, int chunk_len, "
-      "bool need_x, bool need_a, bool need_zi) -> ()");
+      "bool need_x, bool need_a, bool need_zi, int[] extents) -> ()");
   m.def("scan(Tensor x, Tensor w, Tensor init, int chunk_len=0) -> Tensor");
   m.def(
       "scan_backward(Tensor grad, Tensor w, Tensor init, Tensor h, bool need_x, bool need_w, "

@@ -54,12 +54,14 @@ class RaggedPlan:
         self.total_samples = int(self.x_off[-1])
         self.total_coeffs = int(self.a_off[-1])
         self.total_state = int(self.z_off[-1])
-        # per order: (N, device table, K, G, max chunks, chunk length)
-        self.groups: List[Tuple[int, torch.Tensor, int, int, int, int]] = []
+        # per order: (N, device table, K, G, max chunks, chunk length, extents);
+        # extents = elements of packed x, A and zi the group's signals reach,
+        # which the launches check the packed tensors against
+        self.groups: List[Tuple[int, torch.Tensor, int, int, int, int, List[int]]] = []
         for n in sorted(set(orders_t.tolist())):
             assert n <= 32, "ragged batches support orders up to 32"
             idx = (orders_t == n).nonzero().flatten()
-            table, G, maxp, L = ops().lpc_ragged_plan(
+            table, G, maxp, L, xe, ae, ze = ops().lpc_ragged_plan(
                 self.x_off[idx].contiguous(),
                 self.a_off[idx].contiguous(),
                 self.z_off[idx].contiguous(),
@@ -68,7 +70,7 @@ class RaggedPlan:
                 dtype,
                 chunk_len,
             )
-            self.groups.append((n, table.to(self.device), int(idx.numel()), G, maxp, L))
+            self.groups.append((n, table.to(self.device), int(idx.numel()), G, maxp, L, [xe, ae, ze]))
         # streams for the groups: longest first onto the least loaded stream,
         # a group's cost taken as its padded order times its samples
         self._streams: Optional[List[torch.cuda.Stream]] = None
@@ -76,7 +78,8 @@ class RaggedPlan:
         load = [0] * nstreams
         self.lanes: List[int] = [0] * len(self.groups)
         cost = []
-        for gi, (n, _, _, _, _, _) in enumerate(self.groups):
+        for gi, grp in enumerate(self.groups):
+            n = grp[0]
             np_ = 1 << (n - 1).bit_length()
             cost.append((np_ * int(lengths_t[orders_t == n].sum()), gi))
         for c, gi in sorted(cost, reverse=True):
@@ -119,11 +122,17 @@ class RaggedLPC(Function):
 
     @staticmethod
     def forward(x: torch.Tensor, a: torch.Tensor, zi: torch.Tensor, plan: RaggedPlan) -> torch.Tensor:
+        for t in (x, a, zi):
+            if t.dtype != plan.dtype or t.device != plan.device:
+                raise ValueError(
+                    f"ragged_lpc: tensors must be {plan.dtype} on {plan.device} as the plan was "
+                    f"made for, got {t.dtype} on {t.device}"
+                )
         y = torch.empty_like(x)
 
         def one(grp):
-            n, table, K, G, maxp, L = grp
-            ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L)
+            n, table, K, G, maxp, L, ext = grp
+            ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L, ext)
 
         plan.run(one)
         return y
@@ -146,9 +155,9 @@ class RaggedLPC(Function):
         if need_x or need_a or need_zi:
 
             def one(grp):
-                n, table, K, G, maxp, L = grp
+                n, table, K, G, maxp, L, ext = grp
                 ops().lpc_ragged_backward_(
-                    g, a, zi, y, table, dx, da, dzi, K, G, maxp, n, L, need_x, need_a, need_zi
+                    g, a, zi, y, table, dx, da, dzi, K, G, maxp, n, L, need_x, need_a, need_zi, ext
                 )
 
             ctx.plan.run(one)
