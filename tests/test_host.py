@@ -130,6 +130,8 @@ def test_autotune_candidates():
     c32 = autotune.default_candidates(8, 5000, 32, torch.float64)
     assert (2, 0) not in c32 and all(L % 32 == 0 for s, L in c32 if s == 1)
     assert autotune.default_candidates(3, 0, 4, torch.float32) == [(0, 0)]
+    assert (2, 0) in autotune.default_candidates(1024, 16384, 32, torch.float32, ti=True)
+    assert (2, 0) not in autotune.default_candidates(1024, 16384, 32, torch.float32)
 
 
 def test_meta_order16_keeps_forward_summaries():
@@ -165,7 +167,9 @@ def test_plan_query():
     for N in (2, 32):
         s, L = plan_of(meta(4, 1 << 20), meta(4, N))
         assert s == 1 and 1254 <= L <= 1280 + 32, (N, L)
-    assert plan_of(meta(1024, 16384), meta(1024, 32)) == (1, 352)
+    # time-invariant order 16..32 over a full batch: the per-signal MFMA kernels
+    assert plan_of(meta(1024, 16384), meta(1024, 32)) == (2, 0)
+    assert plan_of(meta(1024, 16384), meta(1024, 12)) == (1, 352)
     assert plan_of(meta(4, 3000), meta(4, 3000, 16), strategy=1, chunk_len=100)[1] % 16 == 0
 
 

@@ -104,16 +104,17 @@ def clear() -> None:
         _loaded = True
 
 
-def default_candidates(B: int, T: int, N: int, dtype: torch.dtype) -> List[Tuple[int, int]]:
+def default_candidates(B: int, T: int, N: int, dtype: torch.dtype, ti: bool = False) -> List[Tuple[int, int]]:
     """(strategy, chunk_len) pairs worth timing for a batch shape.
 
-    The heuristic (0, 0) always; the per-signal kernels for float32 order 16;
-    the chunked scan at chunk lengths that give between 1/4 and 4 times as many
-    rerun lanes as compute units can hold (chunk lengths are multiples of 16,
-    32 from order 17 up)."""
+    The heuristic (0, 0) always; the per-signal kernels for float32 order 16
+    (time-varying) and orders 16..32 (time-invariant a[B, N]); the chunked scan
+    at chunk lengths that give between 1/4 and 4 times as many rerun lanes as
+    compute units can hold (chunk lengths are multiples of 16, 32 from order 17
+    up)."""
     blk = 32 if N > 16 else 16
     out: List[Tuple[int, int]] = [(STRATEGY_AUTO, 0)]
-    if N == 16 and dtype == torch.float32:
+    if dtype == torch.float32 and (N == 16 or (ti and 16 <= N <= 32)):
         out.append((STRATEGY_SIGNAL, 0))
     if T <= 0:
         return out
@@ -179,7 +180,8 @@ def tune(
     assert x.is_cuda, "tuning needs GPU tensors"
     B, T = x.shape
     N = A.shape[-1]
-    cands = list(candidates) if candidates is not None else default_candidates(B, T, N, x.dtype)
+    ti = A.dim() == 2 or (A.dim() == 3 and A.shape[1] > 1 and A.stride(1) == 0)
+    cands = list(candidates) if candidates is not None else default_candidates(B, T, N, x.dtype, ti)
     if (STRATEGY_AUTO, 0) not in cands:
         cands.insert(0, (STRATEGY_AUTO, 0))
     ms = {c: _time(step_fn(x, A, zi, grad, c), reps) for c in cands}

@@ -128,11 +128,14 @@ inline int pick_chunk_ti(int64_t B, int64_t T, int np, int requested) {
   return (int)((L + blk - 1) / blk * blk);
 }
 
-// The per-signal tiled kernels (one workgroup per signal, every sample of A read
-// once per direction) serve float32, order-16, time-varying filters, forward and
-// backward.
+// The per-signal kernels (one workgroup per signal, no workspace) serve float32
+// filters: order-16 time-varying ones (tiled, every sample of A read once per
+// direction) and time-invariant ones of order 16..32 (block products on MFMA;
+// the reduced da only, not a per-sample dA).
 inline bool signal_ok(int dtype, int n, int flags, int neg) {
-  return dtype == TLPC_F32 && n == 16 && !(flags & TLPC_TIME_INVARIANT) && !neg;
+  if (dtype != TLPC_F32 || neg) return false;
+  if (flags & TLPC_TIME_INVARIANT) return n >= 16 && n <= 32 && !(flags & TLPC_DA_FULL);
+  return n == 16;
 }
 
 // AUTO takes the per-signal kernels when the signals fill the chip: one
@@ -341,6 +344,10 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   }
   if (p.strategy == TLPC_STRATEGY_SIGNAL) {
     if constexpr (sizeof(S) == 4) {
+      if (ti) {
+        a.da_ti = need_da ? static_cast<S*>(dA) : nullptr;
+        return backward ? launch_ti_mfma_bwd(a, st) : launch_ti_mfma_fwd(a, st);
+      }
       a.dA = need_da ? static_cast<S*>(dA) : nullptr;
       return backward ? launch_tile16_bwd(a, st) : launch_tile16_fwd(a, st);
     }

@@ -139,6 +139,14 @@ int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st);
 int launch_tile16_fwd(ChunkArgs<float> a, hipStream_t st);
 int launch_tile16_bwd(ChunkArgs<float> a, hipStream_t st);
 
+// Per-signal time-invariant kernels (orders 16..32, float32, coefficients a[B, n]):
+// one workgroup per signal, block products on MFMA, float64 state transitions,
+// no workspace. The backward takes the gradient in a.x, writes a.dx, a.dzi and
+// the reduced a.da_ti (each optional), and reads a.y only for da.
+bool ti_mfma_ok(int n, int64_t T);
+int launch_ti_mfma_fwd(ChunkArgs<float> a, hipStream_t st);
+int launch_ti_mfma_bwd(ChunkArgs<float> a, hipStream_t st);
+
 // Coefficient as used by the recurrence: A, or -w for the first-order scan.
 template <typename S>
 __device__ __forceinline__ S coef(const ChunkArgs<S>& a, S v) {
