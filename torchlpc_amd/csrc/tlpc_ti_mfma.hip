@@ -26,7 +26,7 @@
 // that touches the state, float32 MFMA only for the zero-state product H x.
 // h, Psi, Phi and Phi^8 are built per signal in float64: Psi's first 32 rows by
 // the recurrence (one lane per column), the rest by a doubling product, h from
-// its first column and Phi^8 by three squarings.
+// its first column and Phi^8 by three squarings (float64 MFMA).
 //
 // The adjoint of a time-invariant filter is the same filter run backwards in
 // time: lambda'_t' = g'_t' - sum_i a_i lambda'_{t'-i} with g'_t' = g_{T-1-t'} and
@@ -151,26 +151,22 @@ __device__ __forceinline__ void load_rows32(double (&ma)[16], double (&mb)[16], 
   }
 }
 
-// 32x32 double product C = A B of row-major matrices in LDS (pitch MP), all
-// threads: thread t computes rows 2(t/16) + {0,1}, columns 2(t%16) + {0,1}. With
-// MP = 34 the four rows a wave reads fall on distinct banks.
-constexpr int MP = 34;
-__device__ __forceinline__ void matmul32(double* C, const double* A, const double* Bm, int tid) {
-  const int i0 = 2 * (tid >> 4);
-  const int j0 = 2 * (tid & 15);
-  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
-#pragma unroll 8
-  for (int m = 0; m < NP; ++m) {
-    const double a0 = A[i0 * MP + m], a1 = A[(i0 + 1) * MP + m];
-    const double2 bv = *reinterpret_cast<const double2*>(Bm + m * MP + j0);
-    c00 = fma(a0, bv.x, c00);
-    c01 = fma(a0, bv.y, c01);
-    c10 = fma(a1, bv.x, c10);
-    c11 = fma(a1, bv.y, c11);
+// 32x32 double product C = A B on float64 MFMA, wave w computing the 16x16 tile
+// (w / 2, w % 2); row m of B is read at Bm + m * bstep (bstep may be negative,
+// to walk rows backwards). Pitches in doubles.
+__device__ __forceinline__ void matmul32(double* C, int cp, const double* A, int ap, const double* Bm, int bstep,
+                                         int wave, int l16, int lq) {
+  const int ti = wave >> 1, tj = wave & 1;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < NP / 4; ++s) {
+    const int k = 4 * s + lq;
+    acc = mfma16d(A[(16 * ti + l16) * ap + k], Bm[k * bstep + 16 * tj + l16], acc);
   }
-  *reinterpret_cast<double2*>(C + i0 * MP + j0) = make_double2(c00, c01);
-  *reinterpret_cast<double2*>(C + (i0 + 1) * MP + j0) = make_double2(c10, c11);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C[(16 * ti + lq + 4 * r) * cp + 16 * tj + l16] = acc[r];
 }
+constexpr int MP = 34;  // pitch of the Phi-power temporaries
 
 // The MFMA A operands of the output products are fixed per wave: wave w owns
 // row tiles (RB0, RB1) = (0, 3) or (1, 2) (both 20 k-steps of H) of column tile
@@ -269,20 +265,9 @@ __global__ __launch_bounds__(NT, 2) void ti_mfma(ChunkArgs<float> a) {
     }
   }
   __syncthreads();
-  {
-    // rows 32..63: Psi[32+t][k] = sum_m Psi[t][m] Psi[31-m][k] (the state after
-    // 32 steps is s[m] = y_{31-m})
-    const int t = tid >> 3;
-    const int k0 = (tid & 7) * 4;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int m = 0; m < NP; ++m) {
-      const double pv = psi[t * DP + m];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = fma(pv, psi[(NP - 1 - m) * DP + k0 + q], acc[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) psi[(NP + t) * DP + k0 + q] = acc[q];
-  }
+  // rows 32..63: Psi[32+t][k] = sum_m Psi[t][m] Psi[31-m][k] (the state after
+  // 32 steps is s[m] = y_{31-m})
+  matmul32(psi + NP * DP, DP, psi, DP, psi + (NP - 1) * DP, -DP, wave, l16, lq);
   __syncthreads();  // h reads rows 32..62
   if (tid < 128) {
     // h_0 = 1, h_t = Psi[t-1][0]: the impulse leaves the unit state e_0 behind it
@@ -301,11 +286,11 @@ __global__ __launch_bounds__(NT, 2) void ti_mfma(ChunkArgs<float> a) {
     phiA[r * MP + k] = v;
   }
   __syncthreads();
-  matmul32(phiB, phiA, phiA, tid);  // Phi^2
+  matmul32(phiB, MP, phiA, MP, phiA, MP, wave, l16, lq);  // Phi^2
   __syncthreads();
-  matmul32(phiA, phiB, phiB, tid);  // Phi^4
+  matmul32(phiA, MP, phiB, MP, phiB, MP, wave, l16, lq);  // Phi^4
   __syncthreads();
-  matmul32(phiB, phiA, phiA, tid);  // Phi^8
+  matmul32(phiB, MP, phiA, MP, phiA, MP, wave, l16, lq);  // Phi^8
   __syncthreads();
   for (int e = tid; e < NP * NP; e += NT) ph8[(e >> 5) * DP + (e & 31)] = phiB[(e >> 5) * MP + (e & 31)];
   // chains in 32-lane halves: lane position cl, row half ch, state element ce
