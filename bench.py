@@ -395,6 +395,32 @@ def bench_workload(name, steps, warmup, ragged_order=0):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     gbs = alg / (ms * 1e-3) / 1e9
+    extra = {}
+    if name == "b4-t1048576-n2":
+        # the same step with the adjoint that rebuilds y from the forward's chunk
+        # carries (what the autograd Function runs at this length): time, and the
+        # bytes kept between the passes instead of y
+        def run_rc():
+            y, ws = ops.lpc_carries(x, A, zi, 0, 0)
+            ops.lpc_backward_recompute(w, A, zi, x, ws, True, True, True, 0, 0)
+            return ws
+
+        for _ in range(warmup):
+            ws = run_rc()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            ws = run_rc()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_rc = e0.elapsed_time(e1) / steps
+        extra["rebuild_y"] = {
+            "ms_per_step": ms_rc,
+            "Msamples_per_s": S / (ms_rc * 1e-3) / 1e6,
+            "kept_bytes": int(ws.numel()),
+            "y_bytes": int(x.numel() * x.element_size()),
+            "memory_saved_bytes": int(x.numel() * x.element_size() - ws.numel()),
+        }
     return {
         "metric": "Msamples/s (secondary workload)",
         "value": S / (ms * 1e-3) / 1e6,
@@ -409,6 +435,7 @@ def bench_workload(name, steps, warmup, ragged_order=0):
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg},
+        **extra,
     }
 
 

@@ -122,6 +122,27 @@ int tlpc_lpc_backward(int dtype, const void* g, const void* A, const void* zi, c
                       size_t workspace_bytes, void* hip_stream);
 
 /*
+ * The backward of tlpc_lpc_backward without the forward's output y: the adjoint
+ * rerun rebuilds y chunk by chunk from the forward input x and the chunk
+ * carries that tlpc_lpc_forward left in its workspace (`fwd_workspace`, the
+ * whole forward workspace of the same dtype, B, T, N, strategy and chunk_len,
+ * untouched since). The results equal tlpc_lpc_backward's given the forward's
+ * y. `workspace` is the backward's own (tlpc_lpc_backward_workspace). For long
+ * signals this keeps B*P*N float64 carries between the passes instead of B*T
+ * samples of y. Served when tlpc_lpc_recompute_supported() returns 1: real
+ * dtypes, time-varying A, N <= 4, the chunked strategy with chunks of at most
+ * 256 samples (TLPC_EINVAL otherwise). Replaces the saved output of
+ * torchlpc/core.py:179-184 and its read in core.py:222-227.
+ */
+int tlpc_lpc_recompute_supported(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,
+                                 int32_t strategy, int32_t chunk_len);
+int tlpc_lpc_backward_recompute(int dtype, const void* g, const void* A, const void* zi,
+                                const void* x, const void* fwd_workspace, size_t fwd_workspace_bytes,
+                                void* dx, void* dA, void* dzi, int64_t B, int64_t T, int32_t N,
+                                int32_t flags, int32_t strategy, int32_t chunk_len, void* workspace,
+                                size_t workspace_bytes, void* hip_stream);
+
+/*
  * The plan that tlpc_lpc_forward (backward = 0) or tlpc_lpc_backward (backward =
  * 1) runs for these arguments: the resolved strategy (TLPC_STRATEGY_*, never
  * AUTO) and the chunk length (0 for the per-signal strategy, whose tiles are

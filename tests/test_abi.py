@@ -131,3 +131,23 @@ def test_argument_validation_without_launch(lib):
     assert lib.tlpc_lpc_forward(TLPC_F32, None, None, buf, None, 2, 0, 4, 0, 0, 0, None, 0, None) == 0
     assert lib.tlpc_lpc_forward(TLPC_F32, None, None, buf, None, 2, 0, 4, TI, 0, 0, None, 0, None) == TLPC_EINVAL
     assert lib.tlpc_scan(TLPC_F32, None, None, buf, None, 2, 0, 0, None, 0, None) == 0
+
+
+def test_recompute_queries_and_validation(lib):
+    i64, i32, p, sz = ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t
+    lib.tlpc_lpc_recompute_supported.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32]
+    f = lib.tlpc_lpc_backward_recompute
+    f.argtypes = [ctypes.c_int, p, p, p, p, p, sz, p, p, p, i64, i64, i32, i32, i32, i32, p, sz, p]
+    assert lib.tlpc_lpc_recompute_supported(TLPC_F32, 4, 1 << 20, 2, NEED_DX | NEED_DA, 0, 0) == 1
+    assert lib.tlpc_lpc_recompute_supported(TLPC_F64, 4, 1 << 20, 4, NEED_DX, 1, 128) == 1
+    assert lib.tlpc_lpc_recompute_supported(TLPC_F32, 4, 1 << 20, 5, NEED_DX, 0, 0) == 0
+    assert lib.tlpc_lpc_recompute_supported(TLPC_F32, 4, 1 << 20, 2, NEED_DX | TI, 0, 0) == 0
+    assert lib.tlpc_lpc_recompute_supported(TLPC_C64, 4, 1 << 20, 2, NEED_DX, 0, 0) == 0
+    assert lib.tlpc_lpc_recompute_supported(TLPC_F32, 4, 1 << 20, 2, NEED_DX, 1, 512) == 0
+    buf = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation first
+    # order 16 is not served
+    assert f(TLPC_F32, buf, buf, buf, buf, buf, 1 << 30, buf, None, None, 2, 4096, 16, NEED_DX, 0, 0, buf, 1 << 30, None) == TLPC_EINVAL
+    # no forward input
+    assert f(TLPC_F32, buf, buf, buf, None, buf, 1 << 30, buf, None, None, 2, 4096, 2, NEED_DX, 0, 0, buf, 1 << 30, None) == TLPC_EINVAL
+    # forward workspace too small
+    assert f(TLPC_F32, buf, buf, buf, buf, buf, 8, buf, None, None, 2, 4096, 2, NEED_DX, 0, 0, buf, 1 << 30, None) == TLPC_EWORKSPACE

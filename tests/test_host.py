@@ -173,6 +173,53 @@ def test_plan_query():
     assert plan_of(meta(4, 3000), meta(4, 3000, 16), strategy=1, chunk_len=100)[1] % 16 == 0
 
 
+def test_recompute_mode_queries_and_selection(monkeypatch):
+    """The y-rebuilding backward is served for long real time-varying signals of
+    order up to 4 (chunks of at most 256 samples), and LPC.apply takes it for
+    signals of RECOMPUTE_MIN_T samples and more: it saves x and the forward's
+    workspace, not y, and the backward calls lpc_backward_recompute."""
+    from torchlpc_amd import core
+    from torchlpc_amd._ext import recompute_supported
+
+    f64 = torch.float64
+    assert recompute_supported(meta(4, 1 << 20), meta(4, 1 << 20, 2))
+    assert recompute_supported(meta(4, 1 << 20, dtype=f64), meta(4, 1 << 20, 2, dtype=f64))
+    assert not recompute_supported(meta(4, 5000), meta(4, 5000, 16))  # order 16
+    assert not recompute_supported(meta(4, 5000), meta(4, 2))  # time-invariant
+    assert not recompute_supported(meta(64, 1 << 20), meta(64, 1 << 20, 2))  # 2048-sample chunks
+    assert recompute_supported(meta(64, 1 << 20), meta(64, 1 << 20, 2), 1, 256)
+    assert not recompute_supported(meta(4, 5000, dtype=torch.complex64), meta(4, 5000, 2, dtype=torch.complex64))
+
+    real = core.ops()
+    calls = []
+
+    class Spy:
+        def __getattr__(self, name):
+            return getattr(real, name)
+
+        def lpc_carries(self, *a):
+            calls.append("carries")
+            return real.lpc_carries(*a)
+
+        def lpc_backward_recompute(self, *a):
+            calls.append("rebuild")
+            return real.lpc_backward_recompute(*a)
+
+    monkeypatch.setattr(core, "ops", lambda: Spy())
+    T = core.RECOMPUTE_MIN_T
+    x, A, zi = meta(2, T).requires_grad_(), meta(2, T, 2).requires_grad_(), meta(2, 2).requires_grad_()
+    saved = []
+    with torch.autograd.graph.saved_tensors_hooks(lambda t: saved.append(t) or t, lambda t: t):
+        y = LPC.apply(x, A, zi)
+    assert calls == ["carries"]
+    assert any(t is x for t in saved) and not any(t is y for t in saved)
+    gx, gA, gz = torch.autograd.grad(y.sum(), (x, A, zi))
+    assert calls[-1] == "rebuild" and gA.shape == A.shape and gz.shape == zi.shape
+    calls.clear()
+    LPC.apply(meta(2, T - 16).requires_grad_(), meta(2, T - 16, 2), meta(2, 2))  # shorter: y is saved
+    assert calls == []
+
+
 def test_keep_decision_follows_the_callers_grad_mode(monkeypatch):
     """LPC.apply keeps the forward's workspace only when a backward can follow: the
     caller's grad mode on and an input that requires grad. The workspace is then a

@@ -18,6 +18,8 @@ _OPS = (
     "lpc",
     "lpc_keep",
     "lpc_backward",
+    "lpc_carries",
+    "lpc_backward_recompute",
     "lpc_ragged_plan",
     "lpc_ragged_",
     "lpc_ragged_backward_",
@@ -53,6 +55,17 @@ def _load() -> None:
 
 
 _DTYPE_CODE = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}
+_HIPLIB = None
+
+
+def _hiplib():
+    """The C-ABI library through ctypes, for its host-side queries (opened once)."""
+    global _HIPLIB
+    if _HIPLIB is None:
+        import ctypes
+
+        _HIPLIB = ctypes.CDLL(str(HIP_LIB))
+    return _HIPLIB
 
 
 def keep_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int, chunk_len: int) -> int:
@@ -63,7 +76,7 @@ def keep_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int, chunk_
     """
     import ctypes
 
-    lib = ctypes.CDLL(str(HIP_LIB))
+    lib = _hiplib()
     i64, i32 = ctypes.c_int64, ctypes.c_int32
     for name in ("tlpc_lpc_summary_bytes", "tlpc_lpc_forward_workspace", "tlpc_lpc_backward_workspace"):
         fn = getattr(lib, name)
@@ -83,6 +96,36 @@ def keep_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int, chunk_
     return max(fwd, bwd)
 
 
+def recompute_supported(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chunk_len: int = 0) -> bool:
+    """Whether the backward can rebuild y from the forward's chunk carries for these
+    inputs (``tlpc_lpc_recompute_supported``, host only)."""
+    import ctypes
+
+    if x.dtype not in (torch.float32, torch.float64) or A.dim() != 3 or (A.shape[1] > 1 and A.stride(1) == 0):
+        return False
+    lib = _hiplib()
+    i64, i32 = ctypes.c_int64, ctypes.c_int32
+    fn = lib.tlpc_lpc_recompute_supported
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32]
+    B, T = x.shape
+    return fn(_DTYPE_CODE[x.dtype], B, T, A.shape[-1], 7, strategy, chunk_len) == 1
+
+
+def forward_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chunk_len: int = 0) -> int:
+    """``tlpc_lpc_forward_workspace`` for these inputs (host only)."""
+    import ctypes
+
+    lib = _hiplib()
+    i64, i32 = ctypes.c_int64, ctypes.c_int32
+    fn = lib.tlpc_lpc_forward_workspace
+    fn.restype = ctypes.c_size_t
+    fn.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32]
+    B, T = x.shape
+    ti = A.dim() == 2 or (A.shape[1] > 1 and A.stride(1) == 0)
+    return int(fn(_DTYPE_CODE[x.dtype], B, T, A.shape[-1], 8 if ti else 0, strategy, chunk_len))
+
+
 def plan_of(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chunk_len: int = 0,
             backward: bool = False) -> tuple[int, int]:
     """(strategy, chunk length) that the forward (or backward) runs for these
@@ -90,7 +133,7 @@ def plan_of(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chunk_len: int 
     nothing. Strategy codes: 1 chunked, 2 per-signal, 3 serial."""
     import ctypes
 
-    lib = ctypes.CDLL(str(HIP_LIB))
+    lib = _hiplib()
     i64, i32 = ctypes.c_int64, ctypes.c_int32
     fn = lib.tlpc_lpc_plan
     fn.restype = ctypes.c_int
@@ -136,6 +179,18 @@ def _register_fake() -> None:
     def _lpc_keep_fake(x, A, zi, strategy=0, chunk_len=0):
         nbytes = keep_workspace_bytes(x, A, strategy, chunk_len)
         return torch.empty_like(x), x.new_empty((nbytes,), dtype=torch.uint8)
+
+    @torch.library.register_fake(lib + "lpc_carries")
+    def _lpc_carries_fake(x, A, zi, strategy=0, chunk_len=0):
+        nbytes = forward_workspace_bytes(x, A, strategy, chunk_len)
+        return torch.empty_like(x), x.new_empty((nbytes,), dtype=torch.uint8)
+
+    @torch.library.register_fake(lib + "lpc_backward_recompute")
+    def _lpc_bwd_rc_fake(g, A, zi, x, fws, need_x, need_a, need_zi, strategy=0, chunk_len=0):
+        dx = torch.empty_like(g) if need_x else g.new_empty((0,))
+        dA = g.new_empty(A.shape) if need_a else g.new_empty((0,))
+        dzi = g.new_empty(zi.shape) if need_zi else g.new_empty((0,))
+        return dx, dA, dzi
 
     @torch.library.register_fake(lib + "lpc_backward")
     def _lpc_bwd_fake(g, A, zi, y, need_x, need_a, need_zi, strategy=0, chunk_len=0, summary=None):

@@ -16,6 +16,13 @@ custom ``vmap``. The arithmetic runs in the HIP kernels of
   coefficients, as the reference does at ``core.py:194-227``) so that they stay
   differentiable.
 
+Long signals (``T >= RECOMPUTE_MIN_T``, real data, order up to 4): the forward
+runs ``lpc_carries`` and the Function saves x and the forward's chunk carries
+instead of y; the backward (``lpc_backward_recompute``) rebuilds y chunk by
+chunk inside its adjoint pass. Between the passes that keeps B*P*N float64
+carries (P = T / chunk length) instead of B*T samples of y; the reference saves
+y (``torchlpc/core.py:179-184``).
+
 Besides ``A[B,T,N]`` the Function accepts a time-invariant ``a[B,N]`` (its
 gradient is then ``[B,N]``); a time-broadcast ``A`` (``A.stride(1) == 0``) is
 read without being materialised.
@@ -95,6 +102,26 @@ def _keep_summaries(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> bool:
     return x.requires_grad or A.requires_grad or zi.requires_grad
 
 
+# Signals at least this long keep the forward's chunk carries instead of y when
+# the backward can rebuild y from them (set to a huge value to always save y).
+RECOMPUTE_MIN_T = 1 << 16
+
+
+def _recompute_plan(x: torch.Tensor, A: torch.Tensor, plan: Tuple[int, int]) -> Optional[Tuple[int, int]]:
+    """The (strategy, chunk length) of a y-rebuilding backward for this forward, or
+    None: long time-varying real signals of order up to 4 whose chunks the
+    rebuilding rerun takes (at most 256 samples, so a plan with longer chunks
+    is replaced by 256-sample chunks)."""
+    from ._ext import recompute_supported
+
+    if x.shape[1] < RECOMPUTE_MIN_T or A.dim() != 3 or A.shape[-1] > 4 or x.dtype not in (torch.float32, torch.float64):
+        return None
+    for cand in (plan, (1, 256)):
+        if recompute_supported(x, A, *cand):
+            return cand
+    return None
+
+
 class LPC(Function):
     """y[t] = x[t] - sum_{i=1..N} A[t,i] y[t-i], with y[-k] = zi[k-1]."""
 
@@ -107,24 +134,35 @@ class LPC(Function):
     def forward(x: torch.Tensor, A: torch.Tensor, zi: torch.Tensor) -> torch.Tensor:
         strategy, chunk = autotune.lookup("lpc_forward", x, A)
         if _keep_summaries(x, A, zi):
+            rc = _recompute_plan(x, A, (strategy, chunk))
+            if rc is not None:
+                # keep the chunk carries; the backward rebuilds y from them
+                y, ws = ops().lpc_carries(x, A, zi, *rc)
+                _kept.item = (y, ws, rc[0], rc[1], True)
+                return y
             # the backward runs with the forward's strategy and chunk length, so
             # that the kept summaries match its chunking
             y, ws = ops().lpc_keep(x, A, zi, strategy, chunk)
-            _kept.item = (y, ws, strategy, chunk) if ws.numel() > 0 else None
+            _kept.item = (y, ws, strategy, chunk, False) if ws.numel() > 0 else None
             return y
         return ops().lpc(x, A, zi, strategy, chunk)
 
     @staticmethod
     def setup_context(ctx: Any, inputs: List[Any], output: Any) -> Any:
-        _, A, zi = inputs
+        x, A, zi = inputs
         kept = getattr(_kept, "item", None)
         _kept.item = None
         ctx.kept_plan = None
+        ctx.recompute = False
         if kept is not None and kept[0].data_ptr() == output.data_ptr():
             # the workspace is a saved tensor like the others, so saved-tensor
             # hooks (offloading, checkpointing) see it
-            ctx.kept_plan = kept[2:]
-            ctx.save_for_backward(A, zi, output, kept[1])
+            ctx.kept_plan = kept[2:4]
+            ctx.recompute = kept[4]
+            if ctx.recompute:
+                ctx.save_for_backward(A, zi, x, kept[1])
+            else:
+                ctx.save_for_backward(A, zi, output, kept[1])
         else:
             ctx.save_for_backward(A, zi, output)
         ctx.save_for_forward(A, zi, output)
@@ -133,8 +171,18 @@ class LPC(Function):
     def backward(
         ctx: Any, grad_y: torch.Tensor
     ) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor], Optional[torch.Tensor]]:
-        A, zi, y = ctx.saved_tensors[:3]
         need_x, need_A, need_zi = ctx.needs_input_grad[:3]
+        if getattr(ctx, "recompute", False):
+            A, zi, x, ws = ctx.saved_tensors
+            strategy, chunk = ctx.kept_plan
+            if torch.is_grad_enabled():
+                y = ops().lpc(x, A, zi, strategy, chunk)
+                return composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi)
+            dx, dA, dzi = ops().lpc_backward_recompute(
+                grad_y.contiguous(), A, zi, x, ws, need_x, need_A, need_zi, strategy, chunk
+            )
+            return (dx if need_x else None, dA if need_A else None, dzi if need_zi else None)
+        A, zi, y = ctx.saved_tensors[:3]
         if torch.is_grad_enabled():
             return composite_backward(grad_y, A, zi, y, need_x, need_A, need_zi)
         kept_plan = getattr(ctx, "kept_plan", None)

@@ -19,6 +19,7 @@ namespace tlpc {
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr int kRecomputeLmax = 256;  // longest chunk of the y-rebuilding adjoint rerun
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 inline int compiled_order(int n) {
@@ -289,13 +290,43 @@ int dispatch_np(const Plan& p, const ChunkArgs<S>& a, bool backward, hipStream_t
   }
 }
 
+// Adjoint without a saved y: the forward's input and its whole workspace, whose
+// chunk carries the adjoint rerun rebuilds y from.
+struct Recompute {
+  const void* x;
+  const void* fws;
+  size_t fws_bytes;
+};
+
+// Whether the backward with these arguments can rebuild y from the forward's
+// carries: the chunked scan over a time-varying filter of order up to 4 in real
+// data, with chunks the rebuilding rerun takes, planned like the forward.
+inline bool recompute_ok(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
+                         int32_t chunk_len) {
+  if (dtype != TLPC_F32 && dtype != TLPC_F64) return false;
+  if (flags & TLPC_TIME_INVARIANT) return false;
+  if (B < 1 || T < 1 || n < 1 || n > 4) return false;
+  const Plan b = make_plan(dtype, B, T, n, flags, strategy, chunk_len, true);
+  const Plan f = make_plan(dtype, B, T, n, 0, strategy, chunk_len, false);
+  return b.strategy == TLPC_STRATEGY_CHUNKED && f.strategy == TLPC_STRATEGY_CHUNKED && b.L == f.L &&
+         b.P == f.P && b.L <= kRecomputeLmax && b.L % 16 == 0;
+}
+
 template <typename S>
 int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* dx, void* dA,
         void* dzi, int64_t B, int64_t T, int32_t n, int32_t flags, int32_t strategy,
-        int32_t chunk_len, void* ws, size_t ws_bytes, void* stream, bool backward, int neg) {
+        int32_t chunk_len, void* ws, size_t ws_bytes, void* stream, bool backward, int neg,
+        const Recompute* rc = nullptr) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const Plan p = make_plan(dtype, B, T, n, flags, strategy, chunk_len, backward, neg);
   if (B == 0) return TLPC_OK;
+  size_t fcarry_off = 0;
+  if (rc) {
+    if (!backward || !rc->x || !recompute_ok(dtype, B, T, n, flags, strategy, chunk_len)) return TLPC_EINVAL;
+    const Plan f = make_plan(dtype, B, T, n, 0, strategy, chunk_len, false);
+    if (!rc->fws || rc->fws_bytes < f.total) return TLPC_EWORKSPACE;
+    fcarry_off = f.carry_off;
+  }
   const bool ti = (flags & TLPC_TIME_INVARIANT) != 0;
   const bool da_reduced = ti && !(flags & TLPC_DA_FULL);
   const bool need_dx = backward && (flags & TLPC_NEED_DX);
@@ -306,7 +337,7 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
   const bool none = T == 0;
   if ((!A && !(none && !ti)) || (!x && !none)) return TLPC_EINVAL;
   if (!backward && !y && !none) return TLPC_EINVAL;
-  if (backward && need_da && !y && !none) return TLPC_EINVAL;
+  if (backward && need_da && !y && !rc && !none) return TLPC_EINVAL;
   if ((need_dx && !dx && !none) || (need_da && !dA && !(none && !da_reduced)) ||
       (need_dzi && !dzi))
     return TLPC_EINVAL;
@@ -354,6 +385,11 @@ int run(int dtype, const void* x, const void* A, const void* zi, void* y, void* 
     return TLPC_EINVAL;
   }
   using W = typename Wide<S>::type;
+  if (rc) {
+    a.rx = static_cast<const S*>(rc->x);
+    a.fcarry = reinterpret_cast<const W*>(static_cast<const char*>(rc->fws) + fcarry_off);
+    a.y = nullptr;
+  }
   a.summ = reinterpret_cast<W*>(w + p.summ_off);
   a.carry = reinterpret_cast<W*>(w + p.carry_off);
   a.wsum = reinterpret_cast<W*>(w + p.wsum_off);
@@ -546,6 +582,30 @@ TLPC_EXPORT int tlpc_lpc_backward(int dtype, const void* g, const void* A, const
                                   void* hip_stream) {
   return run_dtype(dtype, g, A, zi, const_cast<void*>(y), dx, dA, dzi, B, T, N, flags, strategy,
                    chunk_len, workspace, workspace_bytes, hip_stream, true);
+}
+
+TLPC_EXPORT int tlpc_lpc_recompute_supported(int dtype, int64_t B, int64_t T, int32_t N,
+                                             int32_t flags, int32_t strategy, int32_t chunk_len) {
+  if (check_common(dtype, B, T, N) != TLPC_OK) return 0;
+  return recompute_ok(dtype, B, T, N, flags, strategy, chunk_len) ? 1 : 0;
+}
+
+TLPC_EXPORT int tlpc_lpc_backward_recompute(int dtype, const void* g, const void* A, const void* zi,
+                                            const void* x, const void* fwd_workspace,
+                                            size_t fwd_workspace_bytes, void* dx, void* dA, void* dzi,
+                                            int64_t B, int64_t T, int32_t N, int32_t flags,
+                                            int32_t strategy, int32_t chunk_len, void* workspace,
+                                            size_t workspace_bytes, void* hip_stream) {
+  const int rc = check_common(dtype, B, T, N);
+  if (rc != TLPC_OK) return rc;
+  const Recompute r{x, fwd_workspace, fwd_workspace_bytes};
+  if (dtype == TLPC_F32)
+    return run<float>(dtype, g, A, zi, nullptr, dx, dA, dzi, B, T, N, flags & ~TLPC_REUSE_SUMMARY, strategy,
+                      chunk_len, workspace, workspace_bytes, hip_stream, true, 0, &r);
+  if (dtype == TLPC_F64)
+    return run<double>(dtype, g, A, zi, nullptr, dx, dA, dzi, B, T, N, flags & ~TLPC_REUSE_SUMMARY, strategy,
+                       chunk_len, workspace, workspace_bytes, hip_stream, true, 0, &r);
+  return TLPC_EINVAL;
 }
 
 TLPC_EXPORT int tlpc_lpc_plan(int dtype, int64_t B, int64_t T, int32_t N, int32_t flags,

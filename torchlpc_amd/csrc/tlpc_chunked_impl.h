@@ -1169,8 +1169,12 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
   const S* xb = a.x + sp.xo;
   const S* Ab = a.A + sp.ao;
   S* yb = a.y + sp.xo;
-  S* sink = reinterpret_cast<S*>(a.carry + cid * NP);
   constexpr bool BL = NP <= 4;
+  // steps past the end (the last chunk's tail) sit at the clamped time T-1 and
+  // store the output of T-1 there again, so that no store waits on a branch and
+  // the chunk carries stay as the carry pass left them (an adjoint that rebuilds
+  // y reads them)
+  S ylast = Num<S>::zero();
   for (int jb = 0; jb < len; jb += NP) {
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
@@ -1184,7 +1188,9 @@ __global__ __launch_bounds__(256) void fwd_rerun(ChunkArgs<S> a) {
         for (int i = NP; i >= 1; --i)
           acc = Num<W>::fms(acc, convert<W>(row[i - 1]), ring[(j - i) & (NP - 1)]);
         ring[j] = acc;
-        *(live ? yb + t : sink) = convert<S>(acc);
+        const S yv = convert<S>(acc);
+        ylast = live ? yv : ylast;
+        yb[t] = ylast;
       } else if (jb + j < len) {
         const int64_t t = t0 + jb + j;
         S row[NP];
@@ -1365,6 +1371,118 @@ __global__ __launch_bounds__(256) void bwd_rerun(ChunkArgs<S> a) {
 #pragma unroll
     for (int r = 0; r < NP; ++r)
       if (r < a.n) a.dzi[sp.zo + r] = convert<S>(st.ring[NP - 1 - r]);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Adjoint rerun without a saved y (time-varying, uniform batch, orders up to 4,
+// real data): each lane rebuilds its chunk's y from the forward's chunk carry
+// (a.fcarry) and the forward input (a.rx). Pass 1 walks the chunk forwards and
+// keeps the wide-type state at the start of every 16-sample sub-block in LDS;
+// pass 2 takes the sub-blocks last to first, re-filters each one's 16 outputs
+// into registers (the same operations as fwd_rerun, so the same values) and
+// runs the adjoint steps of bwd_steps over them. 64-lane workgroups; chunks of
+// at most RC_LMAX samples.
+constexpr int RC_SB = 16;
+constexpr int RC_LMAX = 256;
+
+template <typename S, int NP, int JJ>
+__device__ __forceinline__ void rc_fwd_steps(typename Wide<S>::type (&f)[NP], S (&yl)[RC_SB + NP],
+                                             S (&rows)[RC_SB][NP], const ChunkArgs<S>& a, const S* xb,
+                                             const S* Ab, int64_t tq, int64_t T) {
+  using W = typename Wide<S>::type;
+  if constexpr (JJ < RC_SB) {
+    constexpr int j = JJ % NP;  // tq is a multiple of NP
+    const int64_t t = tq + JJ < T ? tq + JJ : T - 1;
+    load_row_u<S, NP>(rows[JJ], a, Ab + t * a.a_tstride);
+    W acc = convert<W>(xb[t]);
+#pragma unroll
+    for (int i = NP; i >= 1; --i) acc = Num<W>::fms(acc, convert<W>(rows[JJ][i - 1]), f[(j - i) & (NP - 1)]);
+    f[j] = acc;
+    yl[NP + JJ] = convert<S>(acc);
+    rc_fwd_steps<S, NP, JJ + 1>(f, yl, rows, a, xb, Ab, tq, T);
+  }
+}
+
+template <typename S, int NP, int JJ>
+__device__ __forceinline__ void rc_adj_steps(typename Wide<S>::type (&ring)[NP], const S (&yl)[RC_SB + NP],
+                                             const S (&rows)[RC_SB][NP], const ChunkArgs<S>& a, const S* gb,
+                                             S* dxb, S* dAb, S* sink, int64_t tq, int64_t T) {
+  using W = typename Wide<S>::type;
+  if constexpr (JJ < RC_SB) {
+    constexpr int s = RC_SB - 1 - JJ;  // sub-block position, last first
+    constexpr int j = s % NP;
+    const int64_t t = tq + s;
+    const bool live = t < T;
+    const int64_t tc = live ? t : T - 1;
+    const S (&row)[NP] = rows[s];  // the rows the rebuild of y just loaded
+    const W lam0 = convert<W>(gb[tc]) + ring[j];
+    const W lam = live ? lam0 : Num<W>::zero();
+    ring[j] = live ? Num<W>::zero() : ring[j];
+#pragma unroll
+    for (int i = 1; i <= NP; ++i)
+      ring[(j - i) & (NP - 1)] = Num<W>::fms(ring[(j - i) & (NP - 1)], Num<W>::conj(convert<W>(row[i - 1])), lam);
+    if (dxb) *(live ? dxb + t : sink) = convert<S>(lam);
+    if (dAb) {
+      S* drow = live ? dAb + t * a.n : sink;
+#pragma unroll
+      for (int i = 1; i <= NP; ++i)
+        if (i <= a.n) drow[i - 1] = coef(a, convert<S>(-(lam * convert<W>(yl[NP + s - i]))));
+    }
+    rc_adj_steps<S, NP, JJ + 1>(ring, yl, rows, a, gb, dxb, dAb, sink, tq, T);
+  }
+}
+
+template <typename S, int NP>
+__global__ __launch_bounds__(64) void bwd_rerun_rc(ChunkArgs<S> a) {
+  using W = typename Wide<S>::type;
+  __shared__ W sbs[(RC_LMAX / RC_SB) * NP * 64];
+  const int lane = threadIdx.x;
+  const int64_t cid = (int64_t)blockIdx.x * 64 + lane;
+  if (cid >= a.B * a.P) return;
+  const int64_t b = cid / a.P;
+  const int64_t c = cid - b * a.P;
+  const int64_t t0 = c * a.L;
+  const int64_t T = a.T;
+  const int nq = a.L / RC_SB;
+  const S* xb = a.rx + b * T;
+  const S* gb = a.x + b * T;  // upstream gradient
+  const S* Ab = a.A + b * a.a_bstride;
+  S* dxb = a.dx ? a.dx + b * T : nullptr;
+  S* dAb = a.dA ? a.dA + b * a.a_bstride : nullptr;
+  S* sink = reinterpret_cast<S*>(a.carry + cid * NP);  // already read; rewritten by the next carry pass
+  // pass 1: the forward state at each sub-block start (slot u mod NP holds y_u)
+  W f[NP];
+  const W* fc = a.fcarry + cid * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) f[NP - 1 - r] = fc[r];
+  S yl[RC_SB + NP];
+  S rows[RC_SB][NP];
+  for (int q = 0; q < nq; ++q) {
+#pragma unroll
+    for (int r = 0; r < NP; ++r) sbs[(q * NP + r) * 64 + lane] = f[r];
+    rc_fwd_steps<S, NP, 0>(f, yl, rows, a, xb, Ab, t0 + (int64_t)q * RC_SB, T);
+  }
+  // pass 2: adjoint over the sub-blocks, last first, from the adjoint carry
+  W ring[NP];
+  const W* cr = a.carry + cid * NP;
+#pragma unroll
+  for (int r = 0; r < NP; ++r) ring[NP - 1 - r] = cr[r];
+  for (int q = nq - 1; q >= 0; --q) {
+    const int64_t tq = t0 + (int64_t)q * RC_SB;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) f[r] = sbs[(q * NP + r) * 64 + lane];
+    // y_{tq-NP+k} = f[k] before the sub-block (y_{-1-k} = zi[k] for chunk 0: the
+    // forward's first carry), then the sub-block's own outputs
+#pragma unroll
+    for (int k = 0; k < NP; ++k) yl[k] = convert<S>(f[k]);
+    rc_fwd_steps<S, NP, 0>(f, yl, rows, a, xb, Ab, tq, T);
+    rc_adj_steps<S, NP, 0>(ring, yl, rows, a, gb, dxb, dAb, sink, tq, T);
+  }
+  if (c == 0 && a.dzi) {
+#pragma unroll
+    for (int r = 0; r < NP; ++r)
+      if (r < a.n) a.dzi[b * a.n + r] = convert<S>(ring[NP - 1 - r]);
   }
 }
 
@@ -1989,6 +2107,16 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
   if (!blocked)
     hipLaunchKernelGGL((carry_pipe<S, NP, false>), dim3((unsigned)a.B), dim3(256), 0, st, a);
   const int64_t nch = chunk_count(a);
+  if (a.rx) {
+    // y rebuilt from the forward's carries (real data, orders up to 4)
+    if constexpr (NP <= 4 && (std::is_same<S, float>::value || std::is_same<S, double>::value)) {
+      if (a.rg || a.dapart || a.L > RC_LMAX || a.L % RC_SB != 0) return -1;
+      hipLaunchKernelGGL((bwd_rerun_rc<S, NP>), dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, st, a);
+      return hipGetLastError() == hipSuccess ? 0 : -4;
+    } else {
+      return -1;
+    }
+  }
   const unsigned rt = rerun_threads(nch);
   const dim3 g((unsigned)((nch + rt - 1) / rt));
   bool tv4 = false;

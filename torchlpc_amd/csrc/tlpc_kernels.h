@@ -50,6 +50,10 @@ struct ChunkArgs {
   // segs (workspace).
   typename Wide<S>::type* segs;
   int64_t nseg;
+  // Adjoint without a saved y (null: y is read): the forward's input and its
+  // chunk carries [B, P, NP], from which the adjoint rerun rebuilds y.
+  const S* rx;
+  const typename Wide<S>::type* fcarry;
 };
 
 // Floats per signal of ChunkArgs::ypad: the y window of an order-NP adjoint

@@ -165,6 +165,70 @@ std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_hip(
   return {dx, dA, dzi};
 }
 
+// Forward that returns its whole workspace: the chunk carries an adjoint without
+// a saved y rebuilds y from (lpc_backward_recompute).
+std::tuple<at::Tensor, at::Tensor> lpc_carries_hip(const at::Tensor& x_, const at::Tensor& A_,
+                                                   const at::Tensor& zi_, int64_t strategy,
+                                                   int64_t chunk_len) {
+  check_inputs(x_, A_, zi_, "lpc_carries");
+  TORCH_CHECK(A_.dim() == 3, "torchlpc_amd::lpc_carries: A must be time-varying [B, T, N]");
+  c10::DeviceGuard guard(x_.device());
+  const auto x = x_.contiguous();
+  const auto zi = zi_.contiguous();
+  const auto A = A_.contiguous();
+  const int64_t B = x.size(0), T = x.size(1), N = A.size(2);
+  const int dt = dtype_code(x);
+  TORCH_CHECK(tlpc_lpc_recompute_supported(dt, B, T, (int32_t)N, TLPC_NEED_DX | TLPC_NEED_DA | TLPC_NEED_DZI,
+                                           (int32_t)strategy, (int32_t)chunk_len) == 1,
+              "torchlpc_amd::lpc_carries: the backward cannot rebuild y for this shape, dtype, "
+              "strategy and chunk length (real data, N <= 4, chunked scan, chunks <= 256)");
+  auto y = at::empty_like(x);
+  const size_t wsb = tlpc_lpc_forward_workspace(dt, B, T, (int32_t)N, 0, (int32_t)strategy, (int32_t)chunk_len);
+  auto ws = workspace(x, wsb);
+  check_rc(tlpc_lpc_forward(dt, x.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(), B, T, (int32_t)N, 0,
+                            (int32_t)strategy, (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr, wsb,
+                            stream_of(x)),
+           "lpc forward");
+  return {y, ws};
+}
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_recompute_hip(
+    const at::Tensor& g_, const at::Tensor& A_, const at::Tensor& zi_, const at::Tensor& x_,
+    const at::Tensor& fws, bool need_x, bool need_a, bool need_zi, int64_t strategy,
+    int64_t chunk_len) {
+  check_inputs(g_, A_, zi_, "lpc_backward_recompute");
+  TORCH_CHECK(A_.dim() == 3, "torchlpc_amd::lpc_backward_recompute: A must be time-varying [B, T, N]");
+  TORCH_CHECK(x_.sizes() == g_.sizes() && x_.scalar_type() == g_.scalar_type() && x_.device() == g_.device(),
+              "torchlpc_amd::lpc_backward_recompute: x must be the forward input, shaped like grad");
+  c10::DeviceGuard guard(g_.device());
+  const auto g = g_.contiguous();
+  const auto zi = zi_.contiguous();
+  const auto x = x_.contiguous();
+  const auto A = A_.contiguous();
+  const int64_t B = g.size(0), T = g.size(1), N = A.size(2);
+  at::Tensor dx = need_x ? at::empty_like(g) : at::empty({0}, g.options());
+  at::Tensor dA = need_a ? at::empty({B, T, N}, g.options()) : at::empty({0}, g.options());
+  at::Tensor dzi = need_zi ? at::empty({B, N}, g.options()) : at::empty({0}, g.options());
+  if (!(need_x || need_a || need_zi)) return {dx, dA, dzi};
+  const int flags = (need_x ? TLPC_NEED_DX : 0) | (need_a ? TLPC_NEED_DA : 0) | (need_zi ? TLPC_NEED_DZI : 0);
+  const int dt = dtype_code(g);
+  const size_t fwb = tlpc_lpc_forward_workspace(dt, B, T, (int32_t)N, 0, (int32_t)strategy, (int32_t)chunk_len);
+  TORCH_CHECK(fws.device() == g.device() && fws.scalar_type() == at::kByte && fws.is_contiguous() &&
+                  (size_t)fws.numel() >= fwb,
+              "torchlpc_amd::lpc_backward_recompute: the workspace does not come from lpc_carries "
+              "with the same shapes, strategy and chunk_len");
+  const size_t wsb = tlpc_lpc_backward_workspace(dt, B, T, (int32_t)N, flags, (int32_t)strategy,
+                                                 (int32_t)chunk_len);
+  auto ws = workspace(g, wsb);
+  check_rc(tlpc_lpc_backward_recompute(dt, g.data_ptr(), A.data_ptr(), zi.data_ptr(), x.data_ptr(),
+                                       fws.data_ptr(), (size_t)fws.numel(), need_x ? dx.data_ptr() : nullptr,
+                                       need_a ? dA.data_ptr() : nullptr, need_zi ? dzi.data_ptr() : nullptr, B,
+                                       T, (int32_t)N, flags, (int32_t)strategy, (int32_t)chunk_len,
+                                       wsb ? ws.data_ptr() : nullptr, wsb, stream_of(g)),
+           "lpc backward (rebuilding y)");
+  return {dx, dA, dzi};
+}
+
 void check_scan(const at::Tensor& x, const at::Tensor& w, const at::Tensor& init, const char* op) {
   TORCH_CHECK(x.is_cuda(), "torchlpc_amd::", op,
               ": tensors must be on a HIP (MI355X) device; CPU execution is not provided");
@@ -342,6 +406,16 @@ void lpc_ragged_backward_hip(const at::Tensor& g, const at::Tensor& A, const at:
 at::Tensor lpc_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t, int64_t) {
   cpu_unsupported("lpc");
 }
+std::tuple<at::Tensor, at::Tensor> lpc_carries_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&,
+                                                   int64_t, int64_t) {
+  cpu_unsupported("lpc_carries");
+}
+std::tuple<at::Tensor, at::Tensor, at::Tensor> lpc_backward_recompute_cpu(const at::Tensor&, const at::Tensor&,
+                                                                          const at::Tensor&, const at::Tensor&,
+                                                                          const at::Tensor&, bool, bool, bool,
+                                                                          int64_t, int64_t) {
+  cpu_unsupported("lpc_backward_recompute");
+}
 std::tuple<at::Tensor, at::Tensor> lpc_keep_cpu(const at::Tensor&, const at::Tensor&,
                                                 const at::Tensor&, int64_t, int64_t) {
   cpu_unsupported("lpc_keep");
@@ -384,6 +458,13 @@ TORCH_LIBRARY(torchlpc_amd, m) {
       "bool need_zi, int strategy=0, int chunk_len=0, Tensor? summary=None) -> "
       "(Tensor, Tensor, Tensor)");
   m.def(
+      "lpc_carries(Tensor x, Tensor A, Tensor zi, int strategy=0, int chunk_len=0) -> "
+      "(Tensor, Tensor)");
+  m.def(
+      "lpc_backward_recompute(Tensor grad_y, Tensor A, Tensor zi, Tensor x, Tensor fwd_workspace, "
+      "bool need_x, bool need_a, bool need_zi, int strategy=0, int chunk_len=0) -> "
+      "(Tensor, Tensor, Tensor)");
+  m.def(
       "lpc_ragged_plan(Tensor x_off, Tensor a_off, Tensor z_off, Tensor lengths, int N, "
       "ScalarType dtype, int chunk_len=0) -> (Tensor, int, int, int, int, int, int)");
   m.def(
@@ -403,6 +484,8 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CUDA, m) {
   m.impl("lpc", &lpc_forward_hip);
   m.impl("lpc_keep", &lpc_keep_hip);
   m.impl("lpc_backward", &lpc_backward_hip);
+  m.impl("lpc_carries", &lpc_carries_hip);
+  m.impl("lpc_backward_recompute", &lpc_backward_recompute_hip);
   m.impl("lpc_ragged_", &lpc_ragged_hip);
   m.impl("lpc_ragged_backward_", &lpc_ragged_backward_hip);
   m.impl("scan", &scan_hip);
@@ -413,6 +496,8 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CPU, m) {
   m.impl("lpc", &lpc_cpu);
   m.impl("lpc_keep", &lpc_keep_cpu);
   m.impl("lpc_backward", &lpc_backward_cpu);
+  m.impl("lpc_carries", &lpc_carries_cpu);
+  m.impl("lpc_backward_recompute", &lpc_backward_recompute_cpu);
   m.impl("lpc_ragged_plan", &lpc_ragged_plan_cpu);
   m.impl("lpc_ragged_", &lpc_ragged_cpu);
   m.impl("lpc_ragged_backward_", &lpc_ragged_backward_cpu);
