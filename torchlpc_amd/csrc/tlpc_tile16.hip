@@ -469,10 +469,18 @@ __device__ __forceinline__ void grads(const float* lam, const float* ywin, float
     for (int j = 0; j < S / 64; ++j) {
       const int r = lane + 64 * j;
       const int64_t t = t0 + r;
+      // every LDS read of the row first, then the products: the products
+      // interleaved with the reads made the wave wait for LDS once per pair
+      float yv[NP];
+#pragma unroll
+      for (int i = 0; i < NP; ++i) yv[i] = ywin[r + 15 - i];  // y_{t-1-i} at window r+15-i
       const float nl = -lam[r];
+      asm volatile("" : "+v"(yv[0]), "+v"(yv[1]), "+v"(yv[2]), "+v"(yv[3]), "+v"(yv[4]), "+v"(yv[5]),
+                   "+v"(yv[6]), "+v"(yv[7]), "+v"(yv[8]), "+v"(yv[9]), "+v"(yv[10]), "+v"(yv[11]),
+                   "+v"(yv[12]), "+v"(yv[13]), "+v"(yv[14]), "+v"(yv[15]));
       float d[NP];
 #pragma unroll
-      for (int i = 0; i < NP; ++i) d[i] = nl * ywin[r + 15 - i];  // y_{t-1-i} at window r+15-i
+      for (int i = 0; i < NP; ++i) d[i] = nl * yv[i];
       if (t < T) {
         float* row = dAb + t * NP;
         if constexpr (ALIGNED) {
