@@ -162,17 +162,25 @@ struct Stage {
 #pragma unroll
     for (int j = 0; j < SA; ++j) a[j] = ld4<ALIGNED>(Ab, t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
   }
+  // Tiles wholly inside the signal (every one but the last) store the staged
+  // values as they are; the last one zeroes what lies past the end.
+  __device__ __forceinline__ bool whole(int64_t T) const { return t0 + S <= T; }
   __device__ __forceinline__ void commit_rows(float4* As, int64_t T, int tid) {
     const int64_t n = T * NP;
+    if (whole(T)) {
 #pragma unroll
-    for (int j = 0; j < SA; ++j) As[tid + STAGERS * j] = zfix(a[j], t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
+      for (int j = 0; j < SA; ++j) As[tid + STAGERS * j] = a[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < SA; ++j) As[tid + STAGERS * j] = zfix(a[j], t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
+    }
   }
   // Forward x of the staged tile (threads past SX load a duplicate).
   __device__ __forceinline__ void load_x(const float* xb, int64_t T, int tid) {
     s0 = ld4<ALIGNED>(xb, t0 + 4 * (tid % SX), T);
   }
   __device__ __forceinline__ void commit_x(float4* xs, int64_t T, int tid) {
-    if (tid < SX) xs[tid] = zfix(s0, t0 + 4 * tid, T);
+    if (tid < SX) xs[tid] = whole(T) ? s0 : zfix(s0, t0 + 4 * tid, T);
   }
   // Backward streams of the staged tile: g, y and the 16-sample y halo below the
   // tile (zi reversed below the signal start: y_{-16+4j+m} = zi[15-4j-m]).
@@ -190,8 +198,13 @@ struct Stage {
     consume(sy);
     consume(sh);
     if (tid < SX) {
-      gs[tid] = zfix(s0, t0 + 4 * tid, T);
-      ys[4 + tid] = zfix(sy, t0 + 4 * tid, T);
+      if (whole(T)) {
+        gs[tid] = s0;
+        ys[4 + tid] = sy;
+      } else {
+        gs[tid] = zfix(s0, t0 + 4 * tid, T);
+        ys[4 + tid] = zfix(sy, t0 + 4 * tid, T);
+      }
     }
     if (tid < 4) {
       const float4 v = zib ? sh : zero4();
