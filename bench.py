@@ -351,7 +351,9 @@ def bench_workload(name, steps, warmup, ragged_order=0):
         sel = [(T, N) for T, N in zip(lengths, orders) if not ragged_order or N == ragged_order]
         S = sum(T for T, _ in sel)
         alg = sum((3 * N + 5) * 4 * T for T, N in sel)
-        workload = f"ragged all-pole filter fwd + fused adjoint bwd, {K} signals, T_k~U{{1024..65536}}, N_k in {{2,4,8,16,32}}, fp32, one HIP graph"
+        workload = (f"ragged all-pole filter fwd + fused adjoint bwd, {K} signals, T_k~U{{1024..65536}}, "
+                    "N_k in {2,4,8,16,32}, fp32, one HIP graph; reflection coefficients k~U(-0.9,0.9) "
+                    "at orders up to 16 and k~U(-0.5,0.5) at order 32 (0.9 overflows float32 there)")
         if ragged_order:
             workload += f", only the {len(sel)} signals of order {ragged_order}"
         cfg = {"workload": workload, "signals": len(sel), "samples": S}
@@ -540,6 +542,8 @@ def main():
                     help="with --workload ragged512: run only the signals of this order (one group)")
     ap.add_argument("--workload", default="b256-t48000-n16", choices=["b256-t48000-n16", "b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32", "ti-b4-t1048576-n32"],
                     help="the other workloads print their own line (the default is B=256, T=48000, N=16)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip timing the secondary workloads after the headline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one copy-like kernel of known size (for counter scaling)")
     ap.add_argument("--trace-summary", metavar="DB", help="summarise a rocprofv3 kernel-trace database and exit")
@@ -749,6 +753,24 @@ def main():
         plain_ms = e0.elapsed_time(e1) / reps
         result["plain_gpu"] = {"ms_per_step": plain_ms, "speedup_fused": plain_ms / ms}
         log(f"[bench] plain GPU form {plain_ms:.3f} ms  (fused speedup {plain_ms / ms:.2f}x)")
+
+    if not args.no_secondary:
+        # the parity workloads, timed in the same run (same rules: inputs resident,
+        # warm-up, event-timed steps on one stream); one short record each
+        sec = {}
+        for name in ("b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32"):
+            log(f"[bench] secondary workload {name}")
+            r = bench_workload(name, 10, 3)
+            sec[name] = {
+                "workload": r["config"]["workload"],
+                "ms_per_step": r["ms_per_step"],
+                "value": r["value"],
+                "unit": r["unit"],
+                "hbm_frac": r["roofline"]["frac"],
+                **({"rebuild_y": r["rebuild_y"]} if "rebuild_y" in r else {}),
+            }
+            torch.cuda.empty_cache()
+        result["secondary"] = sec
 
     if not args.no_cpu_baseline:
         log("[bench] accuracy of the timed plan on a subset of signals")

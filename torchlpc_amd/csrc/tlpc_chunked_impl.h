@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "tlpc_common.h"
+#include "tlpc_dpp16.h"
 #include "tlpc_kernels.h"
 
 namespace tlpc {
@@ -430,8 +431,19 @@ __global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
     }
   };
   const bool own = wave == 0 && lane < NP;
+  // float64 chains of order 16 and 32 take the link's products through DPP row
+  // broadcasts of the state (no LDS round trip per link): order 16 on 16-lane
+  // rows (lane r owns s_r and row r), order 32 on 32-lane halves (lane e = 16h + l
+  // owns s_e and columns 16h..16h+15 of rows l and l + 16); the other lanes of
+  // the wave repeat them. Other orders and types broadcast the state through LDS.
+  constexpr bool kDpp = std::is_same<W, double>::value && (NP == 16 || NP == 32);
+  const int de = NP == 32 ? (lane & 31) : (lane & 15);  // DPP form: the state element of the lane
   W s = Num<W>::zero();
-  if (FWD && own && lane < a.n && a.zi) s = convert<W>(a.zi[sp.zo + lane]);
+  if constexpr (kDpp) {
+    if (FWD && wave == 0 && de < a.n && a.zi) s = convert<W>(a.zi[sp.zo + de]);
+  } else {
+    if (FWD && own && lane < a.n && a.zi) s = convert<W>(a.zi[sp.zo + lane]);
+  }
   const int row = lane < NP ? lane : NP - 1;
   for (int64_t i = -C::DEPTH; i < nbatch; ++i) {
     if (wave > 0) {
@@ -439,6 +451,51 @@ __global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
       if (jn < nbatch && 1 + jn % 3 == wave) fetch(jn);
       const int64_t jc = i + 1;  // commit (fetched DEPTH - 1 iterations ago)
       if (jc >= 0 && jc < nbatch && 1 + jc % 3 == wave) commit(jc);
+    } else if (i >= 0 && kDpp) {
+     if constexpr (kDpp) {
+      const int64_t l0 = i * C::CB;
+      const int nb = (int)((links - l0) < C::CB ? (links - l0) : C::CB);
+      const W(*src)[C::LINK] = buf[i & 1];
+      if constexpr (NP == 32) {
+        const int h = (lane >> 4) & 1, l = lane & 15;
+        double ma[2][16], mb[2][16], zv[2];
+        auto ldm = [&](int li, double(&a0)[16], double(&b0)[16], double& z) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            a0[j] = src[li][l * C::ROWP + 16 * h + j];
+            b0[j] = src[li][(l + 16) * C::ROWP + 16 * h + j];
+          }
+          z = src[li][NP * C::ROWP + de];
+        };
+        ldm(0, ma[0], mb[0], zv[0]);
+#pragma unroll
+        for (int li = 0; li < C::CB; ++li) {
+          if (li + 1 < C::CB) ldm(li + 1, ma[(li + 1) & 1], mb[(li + 1) & 1], zv[(li + 1) & 1]);
+          if (li < nb) {
+            const int64_t l1 = l0 + li;
+            if (own) carry[(FWD ? l1 : P - 1 - l1) * NP + lane] = s;
+            s = dpp16::dpp_matvec32(s, ma[li & 1], mb[li & 1], h) + zv[li & 1];
+          }
+        }
+      } else {
+        double m[2][16], zv[2];
+        auto ldm = [&](int li, double(&mm)[16], double& z) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) mm[k] = src[li][de * C::ROWP + k];
+          z = src[li][NP * C::ROWP + de];
+        };
+        ldm(0, m[0], zv[0]);
+#pragma unroll
+        for (int li = 0; li < C::CB; ++li) {
+          if (li + 1 < C::CB) ldm(li + 1, m[(li + 1) & 1], zv[(li + 1) & 1]);
+          if (li < nb) {
+            const int64_t l1 = l0 + li;
+            if (own) carry[(FWD ? l1 : P - 1 - l1) * NP + lane] = s;
+            s = dpp16::chain_link(s, m[li & 1], zv[li & 1]);
+          }
+        }
+      }
+     }
     } else if (i >= 0) {
       const int64_t l0 = i * C::CB;
       const int nb = (int)((links - l0) < C::CB ? (links - l0) : C::CB);

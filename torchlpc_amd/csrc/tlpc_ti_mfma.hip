@@ -43,10 +43,13 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/torchlpc_amd.h"
+#include "tlpc_dpp16.h"
 #include "tlpc_kernels.h"
 
 namespace tlpc {
 namespace {
+
+using dpp16::dpp_matvec32;
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -106,40 +109,6 @@ __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
 // holds D[4(l/16) + r][l%16]).
 __device__ __forceinline__ f64x4 mfma16d(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// One link of a 32-state chain inside a 32-lane half-wave, without LDS: lane
-// (row half h = (lane >> 4) & 1, position l = lane & 15) holds state element
-// e = 16h + l in `t` and the columns 16h..16h+15 of matrix rows l (`ma`) and
-// l + 16 (`mb`). Each lane forms the partial sums of rows l and l+16 over its
-// half of the columns with v_fmac_f64 reading t through row_newbcast (lane j of
-// its 16-lane row), then swaps one partial with the lane 16 away. Returns
-// sum_m M[e][m] t[m]. Every lane of the wave must take part.
-__device__ __forceinline__ double dpp_matvec32(double t, const double (&ma)[16], const double (&mb)[16], int h) {
-  double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-#define TLPC_DPP2(J0, J1, X0, X1)                                                                        \
-  "v_fmac_f64_dpp %0, %2, %" #X0 " row_newbcast:" #J0 " row_mask:0xf bank_mask:0xf\n\t"                 \
-  "v_fmac_f64_dpp %1, %2, %" #X1 " row_newbcast:" #J1 " row_mask:0xf bank_mask:0xf\n\t"
-  asm volatile("s_nop 1\n\t" TLPC_DPP2(0, 1, 3, 4) TLPC_DPP2(2, 3, 5, 6) TLPC_DPP2(4, 5, 7, 8)
-                   TLPC_DPP2(6, 7, 9, 10) TLPC_DPP2(8, 9, 11, 12) TLPC_DPP2(10, 11, 13, 14)
-                       TLPC_DPP2(12, 13, 15, 16) TLPC_DPP2(14, 15, 17, 18)
-               : "+v"(a0), "+v"(a1)
-               : "v"(t), "v"(ma[0]), "v"(ma[1]), "v"(ma[2]), "v"(ma[3]), "v"(ma[4]), "v"(ma[5]), "v"(ma[6]),
-                 "v"(ma[7]), "v"(ma[8]), "v"(ma[9]), "v"(ma[10]), "v"(ma[11]), "v"(ma[12]), "v"(ma[13]),
-                 "v"(ma[14]), "v"(ma[15]));
-  asm volatile("s_nop 1\n\t" TLPC_DPP2(0, 1, 3, 4) TLPC_DPP2(2, 3, 5, 6) TLPC_DPP2(4, 5, 7, 8)
-                   TLPC_DPP2(6, 7, 9, 10) TLPC_DPP2(8, 9, 11, 12) TLPC_DPP2(10, 11, 13, 14)
-                       TLPC_DPP2(12, 13, 15, 16) TLPC_DPP2(14, 15, 17, 18)
-               : "+v"(b0), "+v"(b1)
-               : "v"(t), "v"(mb[0]), "v"(mb[1]), "v"(mb[2]), "v"(mb[3]), "v"(mb[4]), "v"(mb[5]), "v"(mb[6]),
-                 "v"(mb[7]), "v"(mb[8]), "v"(mb[9]), "v"(mb[10]), "v"(mb[11]), "v"(mb[12]), "v"(mb[13]),
-                 "v"(mb[14]), "v"(mb[15]));
-#undef TLPC_DPP2
-  const double pa = a0 + a1, pb = b0 + b1;
-  // element l (h = 0) takes the upper half's partial of row l; element 16 + l
-  // (h = 1) the lower half's partial of row l + 16
-  const double other = __shfl_xor(h ? pa : pb, 16);
-  return (h ? pb : pa) + other;
 }
 
 // Columns 16h..16h+15 of rows l and l + 16 of a 32x32 double matrix (pitch DP).
