@@ -1,16 +1,18 @@
 // Per-signal tiled filter for order 16, float32 (TLPC_STRATEGY_SIGNAL): forward
 // and fused adjoint, each reading every sample of A, x / g and y from HBM once.
 //
-// One workgroup of 8 waves owns one signal and walks it in tiles of S = 512
-// samples (the backward from the last tile down), C = 16 sub-chunks of LC = 32
-// samples per tile. Each tile passes through three stages, one per period of a
-// software pipeline, so that in period p the waves work on three tiles at once:
+// One workgroup of 8 waves owns one signal and walks it in tiles (the backward
+// from the last tile down) of sub-chunks of LC = 32 samples: 12 sub-chunks (384
+// samples) per tile in the forward, 16 (512 samples) in the backward. Each tile
+// passes through three stages (four in the backward), one per period of a
+// software pipeline, so that in period p the waves work on several tiles at once:
 //
-//   stage 1 (tile p)    waves 0-3: one 16-lane DPP row per sub-chunk sweeps it
-//                       backwards with the transposed recurrence from a unit at
-//                       output r (lane r): row r of the sub-chunk's transition
-//                       matrix Phi, and in the forward z[r] (the lambdas met on
-//                       the way, dotted with x), in float64.
+//   stage 1 (tile p)    one 16-lane DPP row per sub-chunk (forward waves 0-2,
+//                       backward waves 0-3) sweeps it backwards with the
+//                       transposed recurrence from a unit at output r (lane r):
+//                       row r of the sub-chunk's transition matrix Phi, and in
+//                       the forward z[r] (the lambdas met on the way, dotted with
+//                       x), in float64.
 //                       backward, wave 4: one quad per sub-chunk runs the adjoint
 //                       recurrence from a zero state through it (its zero-state
 //                       response w).
@@ -20,15 +22,16 @@
 //                       descending).
 //   stage 3 (tile p-2)  one wave: one quad per sub-chunk reruns it from its carry
 //                       in float64 arithmetic on the float32 data and writes y
-//                       (forward) or dx, the 64-byte dA rows and, at the start of
-//                       the signal, dzi (backward).
+//                       (forward) or lambda into LDS (backward).
+//   stage 4 (tile p-3)  backward, one wave: dx, the 64-byte dA rows and, at the
+//                       start of the signal, dzi.
 //
-// A tile's A rows (32 KB) and per-sample streams sit in LDS from stage 1 to
-// stage 3 (three buffers); the next tile's global loads are issued into
-// registers when a period starts and written to LDS when it ends, so they
-// overlap the period's arithmetic. Stage 1 keeps its summaries in registers
-// until the end of the period, when stage 2 is done with the previous tile's, so
-// one summary buffer serves both.
+// A tile's A rows and per-sample streams sit in LDS from stage 1 to stage 3
+// (three buffers); the next tile's global loads are issued into registers when a
+// period starts and written to LDS when it ends, so they overlap the period's
+// arithmetic. Stage 1 keeps its summaries in registers until the end of the
+// period, when stage 2 is done with the previous tile's, so one summary buffer
+// serves both.
 //
 // Samples past the end of the signal are zero rows and zero inputs in LDS, so
 // every sub-chunk is whole: the forward computes values there that it does not
@@ -63,12 +66,22 @@ constexpr int STAGERS = THREADS;
 constexpr int SX = S / 4;                 // float4 of a per-sample stream per tile
 constexpr int SA = S * NP / 4 / STAGERS;  // float4 of A per staging thread
 static_assert(C == 16 && LC % 16 == 0 && SA * STAGERS * 4 == S * NP && SX <= STAGERS, "tile layout");
+// The forward's tiles hold 12 sub-chunks (384 samples): three waves sweep Phi,
+// one per SIMD, and the fourth SIMD runs the quad rerun and the carry chain,
+// whose serial steps then do not wait behind a Phi wave's issue (the Phi sweep
+// keeps its SIMD's float64 pipe busy). With 16 sub-chunks, four Phi waves and
+// the rerun sharing a SIMD with one of them, a 512-sample period took about
+// 1.65 times a 384-sample one.
+constexpr int FS = 384;
+constexpr int FC = FS / LC;
+constexpr int FSX = FS / 4;
+static_assert(FC % 4 == 0 && FC <= 16 && (FS * NP / 4) % STAGERS == 0 && FSX <= STAGERS, "forward tile layout");
 
 struct FwdLds {
-  float4 A[NBUF][S * 4];  // tile rows: A[b][r*4 + q] = A[t0 + r][4q .. 4q+3]
-  float4 x[NBUF][SX];     // x[t0 .. t0+S)
-  double summ[C][LINKP];  // Phi rows (padded) then z, of the tile in stage 2
-  double carry[2][C][NP]; // state entering each sub-chunk (written in stage 2, read in 3)
+  float4 A[NBUF][FS * 4];  // tile rows: A[b][r*4 + q] = A[t0 + r][4q .. 4q+3]
+  float4 x[NBUF][FSX];     // x[t0 .. t0+FS)
+  double summ[FC][LINKP];  // Phi rows (padded) then z, of the tile in stage 2
+  double carry[2][FC][NP]; // state entering each sub-chunk (written in stage 2, read in 3)
 };
 
 struct BwdLds {
@@ -141,16 +154,18 @@ __device__ __forceinline__ void st4(float* p, int64_t i, int64_t n, float4 v) {
 // staging loads reuse them).
 __device__ __forceinline__ void consume(float4& v) { asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
 
-template <bool ALIGNED>
+template <bool ALIGNED, int TS = S>
 struct Stage {
-  float4 a[SA];
-  float4 s0;      // x (forward) or g (backward): float4 tid % SX of the tile
-  float4 sy, sh;  // backward: y (float4 tid % SX) and the y halo (float4 tid % 4)
+  static constexpr int TSA = TS * NP / 4 / STAGERS;  // float4 of A per staging thread
+  static constexpr int TSX = TS / 4;                 // float4 of a per-sample stream
+  float4 a[TSA];
+  float4 s0;      // x (forward) or g (backward): float4 tid % TSX of the tile
+  float4 sy, sh;  // backward: y (float4 tid % TSX) and the y halo (float4 tid % 4)
   int64_t t0;     // first sample of the staged tile
   // Waits for every staged load; called on every path once per period.
   __device__ __forceinline__ void consume_all() {
 #pragma unroll
-    for (int j = 0; j < SA; ++j) consume(a[j]);
+    for (int j = 0; j < TSA; ++j) consume(a[j]);
     consume(s0);
     consume(sy);
     consume(sh);
@@ -160,34 +175,34 @@ struct Stage {
     t0 = t0_;
     const int64_t n = T * NP;
 #pragma unroll
-    for (int j = 0; j < SA; ++j) a[j] = ld4<ALIGNED>(Ab, t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
+    for (int j = 0; j < TSA; ++j) a[j] = ld4<ALIGNED>(Ab, t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
   }
   // Tiles wholly inside the signal (every one but the last) store the staged
   // values as they are; the last one zeroes what lies past the end.
-  __device__ __forceinline__ bool whole(int64_t T) const { return t0 + S <= T; }
+  __device__ __forceinline__ bool whole(int64_t T) const { return t0 + TS <= T; }
   __device__ __forceinline__ void commit_rows(float4* As, int64_t T, int tid) {
     const int64_t n = T * NP;
     if (whole(T)) {
 #pragma unroll
-      for (int j = 0; j < SA; ++j) As[tid + STAGERS * j] = a[j];
+      for (int j = 0; j < TSA; ++j) As[tid + STAGERS * j] = a[j];
     } else {
 #pragma unroll
-      for (int j = 0; j < SA; ++j) As[tid + STAGERS * j] = zfix(a[j], t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
+      for (int j = 0; j < TSA; ++j) As[tid + STAGERS * j] = zfix(a[j], t0 * NP + 4 * (int64_t)(tid + STAGERS * j), n);
     }
   }
   // Forward x of the staged tile (threads past SX load a duplicate).
   __device__ __forceinline__ void load_x(const float* xb, int64_t T, int tid) {
-    s0 = ld4<ALIGNED>(xb, t0 + 4 * (tid % SX), T);
+    s0 = ld4<ALIGNED>(xb, t0 + 4 * (tid % TSX), T);
   }
   __device__ __forceinline__ void commit_x(float4* xs, int64_t T, int tid) {
-    if (tid < SX) xs[tid] = whole(T) ? s0 : zfix(s0, t0 + 4 * tid, T);
+    if (tid < TSX) xs[tid] = whole(T) ? s0 : zfix(s0, t0 + 4 * tid, T);
   }
   // Backward streams of the staged tile: g, y and the 16-sample y halo below the
   // tile (zi reversed below the signal start: y_{-16+4j+m} = zi[15-4j-m]).
   __device__ __forceinline__ void load_gy(const float* gb, const float* yb, const float* zib, int64_t T,
                                           int tid) {
-    s0 = ld4<ALIGNED>(gb, t0 + 4 * (tid % SX), T);
-    sy = ld4<ALIGNED>(yb, t0 + 4 * (tid % SX), T);
+    s0 = ld4<ALIGNED>(gb, t0 + 4 * (tid % TSX), T);
+    sy = ld4<ALIGNED>(yb, t0 + 4 * (tid % TSX), T);
     // the halo: y below the tile, or zi (or g again, never used, without zi)
     const float* hp = t0 > 0 ? yb : (zib ? zib : gb);
     const int64_t hi = t0 > 0 ? t0 - 16 + 4 * (tid % 4) : 12 - 4 * (tid % 4);
@@ -197,7 +212,7 @@ struct Stage {
     consume(s0);
     consume(sy);
     consume(sh);
-    if (tid < SX) {
+    if (tid < TSX) {
       if (whole(T)) {
         gs[tid] = s0;
         ys[4 + tid] = sy;
@@ -278,7 +293,7 @@ __device__ __forceinline__ void phi_store(double* out, int kk, const double (&p)
 // rows of the wave repeat row 0). Forward: carry[c] = s; s = Phi_c s + z_c for
 // c = 0..C-1. Backward: carry[c] = s; s = Phi_c^T s + w_c for c = C-1..0. The
 // LDS reads of the next link go out before the current link's products.
-template <bool FWD>
+template <bool FWD, int CN = C>
 __device__ __forceinline__ void chain(double& s, const double (*summ)[LINKP], double (*carry)[NP], int lane) {
   const int row = lane & 15;
   double m[2][NP], zv[2];
@@ -287,11 +302,11 @@ __device__ __forceinline__ void chain(double& s, const double (*summ)[LINKP], do
     for (int k = 0; k < NP; ++k) mm[k] = FWD ? summ[c][row * ROWP + k] : summ[c][k * ROWP + row];
     z = summ[c][NP * ROWP + row];
   };
-  ldm(FWD ? 0 : C - 1, m[0], zv[0]);
+  ldm(FWD ? 0 : CN - 1, m[0], zv[0]);
 #pragma unroll
-  for (int i = 0; i < C; ++i) {
-    const int c = FWD ? i : C - 1 - i;
-    if (i + 1 < C) ldm(FWD ? c + 1 : c - 1, m[(i + 1) & 1], zv[(i + 1) & 1]);
+  for (int i = 0; i < CN; ++i) {
+    const int c = FWD ? i : CN - 1 - i;
+    if (i + 1 < CN) ldm(FWD ? c + 1 : c - 1, m[(i + 1) & 1], zv[(i + 1) & 1]);
     if (lane < NP) carry[c][lane] = s;
     s = chain_link(s, m[i & 1], zv[i & 1]);
   }
@@ -343,21 +358,25 @@ __device__ __forceinline__ double fwd_step(double (&R)[4], const float4 c, float
   return y;
 }
 
-template <int JJ>
+template <int JJ, int TS>
 __device__ __forceinline__ void fwd_steps(double (&R)[4], float4 (&cq)[RING], float4& out,
                                           const float4& xc, const float4* ar, int tb, int ql) {
   if constexpr (JJ < 16) {
     const double y = fwd_step<JJ & 3>(R, cq[JJ % RING], qsel<JJ>(xc), ql);
-    cq[JJ % RING] = ar[min(tb + JJ + RING, S - 1) * 4];
+    cq[JJ % RING] = ar[min(tb + JJ + RING, TS - 1) * 4];
     put<JJ>(out, ql, (float)y);
-    fwd_steps<JJ + 1>(R, cq, out, xc, ar, tb, ql);
+    fwd_steps<JJ + 1, TS>(R, cq, out, xc, ar, tb, ql);
   }
 }
 
-template <bool ALIGNED>
+// A tile of TS samples holds TS / LC sub-chunks; quads past them (TS < 16 LC)
+// rerun the last sub-chunk again and store nothing.
+template <bool ALIGNED, int TS>
 __device__ __forceinline__ void fwd_rerun(const float4* As, const float4* xs4, const double (*carry)[NP],
                                           float* yb, int64_t T, int64_t t0, int lane) {
-  const int q = lane >> 2, ql = lane & 3;
+  constexpr int CN = TS / LC;
+  const int qa = lane >> 2, ql = lane & 3;
+  const int q = qa < CN ? qa : CN - 1;
   double R[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) R[3 - m] = carry[q][4 * ql + m];  // y_{t0c-1-4ql-m}, slot 3-m
@@ -369,10 +388,10 @@ __device__ __forceinline__ void fwd_rerun(const float4* As, const float4* xs4, c
 #pragma unroll 1
   for (int jb = 0; jb < LC; jb += 16) {
     const int tb = q * LC + jb;  // row in the tile
-    const float4 xn = xs4[min(tb + 16, S - 16) / 4 + ql];
+    const float4 xn = xs4[min(tb + 16, TS - 16) / 4 + ql];
     float4 out = zero4();
-    fwd_steps<0>(R, cq, out, xc, ar, tb, ql);
-    st4<ALIGNED>(yb, t0 + tb + 4 * ql, T, out);
+    fwd_steps<0, TS>(R, cq, out, xc, ar, tb, ql);
+    if (qa < CN) st4<ALIGNED>(yb, t0 + tb + 4 * ql, T, out);
     xc = xn;
   }
 }
@@ -515,14 +534,17 @@ __device__ __forceinline__ void grads(const float* lam, const float* ywin, float
 }
 
 // ---------------------------------------------------------------------------
-// Roles in period p: waves 0-3 stage 1 (tile p), wave CHAIN_WAVE stage 2 (tile
-// p-1), the rerun wave stage 3 (tile p-2), and in the backward wave DA_WAVE
-// stage 4 (tile p-3); the staging waves load tile p+1. Tile p
+// Roles in period p: the Phi waves stage 1 (tile p), the chain wave stage 2
+// (tile p-1), the rerun wave stage 3 (tile p-2), and in the backward wave
+// DA_WAVE stage 4 (tile p-3); every wave stages tile p+1. Wave w runs on SIMD
+// w % 4. Tile p
 // lives in buffer p % NBUF (the backward's y window in p % (NBUF + 1)), its
 // carries in carry[p % 2].
 constexpr int ZS_WAVE = 4;       // backward: zero-state responses (stage 1)
 constexpr int CHAIN_WAVE = 5;
-constexpr int FWD_RERUN_WAVE = 6;  // forward: rerun and y stores (stage 3)
+constexpr int FWD_PHI_WAVES = FC / 4;  // forward: Phi sweeps (stage 1), waves 0-2
+constexpr int FWD_RERUN_WAVE = 3;      // forward: rerun and y stores (stage 3), SIMD 3
+constexpr int FWD_CHAIN_WAVE = 7;      // forward: carry chain (stage 2), SIMD 3
 constexpr int RERUN_WAVE = 6;    // backward: lambda (stage 3)
 constexpr int DA_WAVE = 7;       // backward: dA and dx stores (stage 4)
 
@@ -533,15 +555,15 @@ __global__ __launch_bounds__(THREADS) void tile_fwd(ChunkArgs<float> a) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t b = blockIdx.x;
   const int64_t T = a.T;
-  const int64_t ntiles = (T + S - 1) / S;
+  const int64_t ntiles = (T + FS - 1) / FS;
   const float* Ab = a.A + b * T * NP;
   const float* xb = a.x + b * T;
   float* yb = a.y + b * T;
   double s = 0.0;  // chain state (lanes 0-15 of the chain wave): y_{t0-1-r}
-  if (wave == CHAIN_WAVE && lane < NP && a.zi) s = (double)a.zi[b * NP + lane];
+  if (wave == FWD_CHAIN_WAVE && lane < NP && a.zi) s = (double)a.zi[b * NP + lane];
   const int si = tid;
   {
-    Stage<ALIGNED> st;
+    Stage<ALIGNED, FS> st;
     st.load_rows(Ab, T, 0, si);
     st.load_x(xb, T, si);
     st.consume_all();
@@ -556,23 +578,23 @@ __global__ __launch_bounds__(THREADS) void tile_fwd(ChunkArgs<float> a) {
     // staged registers live within one period (not carried across the loop)
     // the loads are unconditional (in the last periods they read the last tile
     // again, never committed): a conditional load is waited for at the merge
-    Stage<ALIGNED> st;
-    st.load_rows(Ab, T, min(p + 1, ntiles - 1) * S, si);
+    Stage<ALIGNED, FS> st;
+    st.load_rows(Ab, T, min(p + 1, ntiles - 1) * FS, si);
     st.load_x(xb, T, si);
     double ph[NP], z = 0.0;
     const int kk = lane & 15, c = wave * 4 + (lane >> 4);
-    if (wave < 4 && p < ntiles) {
+    if (wave < FWD_PHI_WAVES && p < ntiles) {
       const int bb = (int)(p % NBUF);
       phi_sweep<true>(L.A[bb], reinterpret_cast<const float*>(L.x[bb]), c, kk, ph, z);
-    } else if (wave == CHAIN_WAVE && p >= 1 && p <= ntiles) {
-      chain<true>(s, L.summ, L.carry[(p - 1) & 1], lane);
+    } else if (wave == FWD_CHAIN_WAVE && p >= 1 && p <= ntiles) {
+      chain<true, FC>(s, L.summ, L.carry[(p - 1) & 1], lane);
     } else if (wave == FWD_RERUN_WAVE && p >= 2) {
       const int64_t k = p - 2;
       const int bb = (int)(k % NBUF);
-      fwd_rerun<ALIGNED>(L.A[bb], L.x[bb], L.carry[k & 1], yb, T, k * S, lane);
+      fwd_rerun<ALIGNED, FS>(L.A[bb], L.x[bb], L.carry[k & 1], yb, T, k * FS, lane);
     }
     lds_barrier();
-    if (wave < 4 && p < ntiles) phi_store<true>(L.summ[c], kk, ph, z);
+    if (wave < FWD_PHI_WAVES && p < ntiles) phi_store<true>(L.summ[c], kk, ph, z);
     st.consume_all();
     if (more) {
       st.commit_rows(L.A[nb], T, si);
