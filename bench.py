@@ -304,7 +304,7 @@ def workload_inputs(name, device, seed):
     raise ValueError(name)
 
 
-def bench_workload(name, steps, warmup, ragged_order=0):
+def bench_workload(name, steps, warmup, ragged_order=0, tune=True):
     """fwd(+bwd) of a secondary workload; returns the JSON record. For the ragged
     workload, ragged_order > 0 runs only that order's group of signals (on the
     caller's stream), for a kernel trace of one group."""
@@ -323,6 +323,9 @@ def bench_workload(name, steps, warmup, ragged_order=0):
         gd = torch.Generator(device=dev).manual_seed(1014)
         w = torch.randn(x.numel(), generator=gd, device=dev, dtype=f32)
         plan = torchlpc_amd.RaggedPlan(lengths, orders, dev, f32)
+        heuristic_chunks = {grp[0]: grp[5] for grp in plan.groups}
+        # untimed: the project tuner picks each order group's chunk length on these inputs
+        tuned = plan.tune(x, A, zi, w) if tune else None
         y = torch.empty_like(x)
         dx, dA, dzi = torch.empty_like(x), torch.empty_like(A), torch.empty_like(zi)
 
@@ -356,7 +359,10 @@ def bench_workload(name, steps, warmup, ragged_order=0):
                     "at orders up to 16 and k~U(-0.5,0.5) at order 32 (0.9 overflows float32 there)")
         if ragged_order:
             workload += f", only the {len(sel)} signals of order {ragged_order}"
-        cfg = {"workload": workload, "signals": len(sel), "samples": S}
+        cfg = {"workload": workload, "signals": len(sel), "samples": S,
+               "chunk_len_by_order": {grp[0]: grp[5] for grp in plan.groups},
+               "heuristic_chunk_len_by_order": heuristic_chunks,
+               "tuned": {n: {"choice": v["choice"], "ms": v["ms"]} for n, v in tuned.items()} if tuned else None}
         s = torch.cuda.Stream()
         s.wait_stream(stream)
         with torch.cuda.stream(s):
@@ -568,7 +574,8 @@ def main():
 
     if args.workload != "b256-t48000-n16":
         log(f"[bench] workload {args.workload}")
-        print(json.dumps(bench_workload(args.workload, args.steps, args.warmup, args.ragged_order)), flush=True)
+        print(json.dumps(bench_workload(args.workload, args.steps, args.warmup, args.ragged_order,
+                                        tune=not args.no_tune)), flush=True)
         return
 
     import torch
@@ -760,9 +767,10 @@ def main():
         sec = {}
         for name in ("b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32"):
             log(f"[bench] secondary workload {name}")
-            r = bench_workload(name, 10, 3)
+            r = bench_workload(name, 10, 3, tune=not args.no_tune)
             sec[name] = {
                 "workload": r["config"]["workload"],
+                **({"chunk_len_by_order": r["config"]["chunk_len_by_order"]} if "chunk_len_by_order" in r["config"] else {}),
                 "ms_per_step": r["ms_per_step"],
                 "value": r["value"],
                 "unit": r["unit"],

@@ -152,3 +152,29 @@ def test_ragged_ops_refuse_tensors_smaller_than_the_plan():
     torch.cuda.synchronize()
     assert torch.equal(y[300:1300], torch.zeros(1000, device=dev0))  # zero input, zero state
     del dx, da, dzi
+
+
+def test_ragged_tune_per_group_chunk_lengths():
+    """RaggedPlan.tune picks a chunk length per order group from timing; the
+    filter then runs with the chosen plans and stays within the tolerance of the
+    FP64 oracle (a chunk boundary changes only float64 rounding)."""
+    from torchlpc_amd import RaggedPlan, ragged_lpc
+
+    rng = np.random.default_rng(23)
+    lengths = [3000, 5000, 700, 4096, 2500, 6000, 1200, 3333]
+    orders = [2, 4, 8, 16, 32, 16, 8, 32]
+    sigs = _mixed(rng, lengths, orders)
+    x, a, zi, g = _pack(sigs, np.float32)
+    X, AA, Z, Gd = dev(x), dev(a), dev(zi), dev(g)
+    plan = RaggedPlan(lengths, orders, X.device, X.dtype)
+    before = {grp[0]: grp[5] for grp in plan.groups}
+    res = plan.tune(X, AA, Z, Gd, reps=2)
+    assert set(res) == set(before)
+    for grp in plan.groups:
+        n, L = grp[0], grp[5]
+        assert L == res[n]["choice"] and L in res[n]["ms"] and before[n] in res[n]["ms"]
+        assert L % (16 if n <= 16 else 32) == 0
+    Xg, Ag, Zg = (t.clone().requires_grad_() for t in (X, AA, Z))
+    y = ragged_lpc(Xg, Ag, plan, Zg)
+    y.backward(Gd)
+    _check(sigs, np.float32, plan, y, Xg.grad, Ag.grad, Zg.grad, REL_TOL[np.float32])

@@ -9,8 +9,9 @@ The reference filters rectangular ``[B, T]`` batches with one order
 
 Each output equals ``sample_wise_lpc`` on the signal alone. A :class:`RaggedPlan`
 holds the metadata (lengths, orders, offsets, and per order a device table that
-maps chunks to signals), built once on the host; the filter then runs as one
-set of chunked-scan launches per order present (summaries, carries, reruns),
+maps chunks to signals), built once on the host (``RaggedPlan.tune`` picks each
+group's chunk length from timing); the filter then runs as one set of
+chunked-scan launches per order present (summaries, carries, reruns),
 forward and fused backward, so a whole ragged step can be captured in one HIP
 graph. The order groups write disjoint parts of the outputs, so they run on up
 to four side streams forked from and joined back into the caller's stream (a
@@ -60,17 +61,7 @@ class RaggedPlan:
         self.groups: List[Tuple[int, torch.Tensor, int, int, int, int, List[int]]] = []
         for n in sorted(set(orders_t.tolist())):
             assert n <= 32, "ragged batches support orders up to 32"
-            idx = (orders_t == n).nonzero().flatten()
-            table, G, maxp, L, xe, ae, ze = ops().lpc_ragged_plan(
-                self.x_off[idx].contiguous(),
-                self.a_off[idx].contiguous(),
-                self.z_off[idx].contiguous(),
-                lengths_t[idx].contiguous(),
-                n,
-                dtype,
-                chunk_len,
-            )
-            self.groups.append((n, table.to(self.device), int(idx.numel()), G, maxp, L, [xe, ae, ze]))
+            self.groups.append(self._group_for(n, chunk_len))
         # streams for the groups: longest first onto the least loaded stream,
         # a group's cost taken as its padded order times its samples
         self._streams: Optional[List[torch.cuda.Stream]] = None
@@ -86,6 +77,83 @@ class RaggedPlan:
             s = min(range(nstreams), key=lambda i: load[i])
             self.lanes[gi] = s
             load[s] += c
+
+    def _group_for(self, n: int, chunk_len: int):
+        """The (N, table, K, G, max chunks, chunk length, extents) group of the
+        signals of order n, chunked at chunk_len (0: the library heuristic)."""
+        idx = (self.orders == n).nonzero().flatten()
+        table, G, maxp, L, xe, ae, ze = ops().lpc_ragged_plan(
+            self.x_off[idx].contiguous(),
+            self.a_off[idx].contiguous(),
+            self.z_off[idx].contiguous(),
+            self.lengths[idx].contiguous(),
+            n,
+            self.dtype,
+            chunk_len,
+        )
+        return (n, table.to(self.device), int(idx.numel()), G, maxp, L, [xe, ae, ze])
+
+    def tune(
+        self,
+        x: torch.Tensor,
+        a: torch.Tensor,
+        zi: torch.Tensor,
+        grad: Optional[torch.Tensor] = None,
+        factors: Sequence[float] = (0.5, 0.75, 1.0, 1.5, 2.0, 3.0),
+        reps: int = 3,
+    ) -> dict:
+        """Chunk length per order group from timing, on the caller's packed tensors.
+
+        For every order group, the group's step (the forward and, with ``grad``,
+        the fused backward) is timed with HIP events at chunk lengths around the
+        library heuristic's (``factors`` times it, rounded to the staging block of
+        the order), and the fastest becomes the group's plan. The chunk length sets
+        the rerun lanes of the group (one lane per chunk) against the length of
+        its carry chains. Returns ``{N: {"choice": L, "ms": {L: ms}}}``.
+        Synchronises the device: not during HIP-graph capture. The reference has
+        no tuner (``torchlpc/csrc/cuda/linear_recurrence.cu:238-239``).
+        """
+        assert x.is_cuda, "tuning needs GPU tensors"
+        y = torch.empty_like(x)
+        if grad is not None:
+            g = grad.contiguous()
+            dx, da, dzi = torch.empty_like(g), torch.empty_like(a), torch.empty_like(zi)
+        out = {}
+        for gi, grp in enumerate(self.groups):
+            n, L0 = grp[0], grp[5]
+            np_ = 1 << (n - 1).bit_length()
+            blk = 16 if np_ < 16 else np_
+            cands = sorted({max(blk, -(-int(L0 * f) // blk) * blk) for f in factors} | {L0})
+            ms = {}
+            best = None
+            for L in cands:
+                cand = grp if L == L0 else self._group_for(n, L)
+
+                def step(c=cand):
+                    n_, table, K, G, maxp, Lc, ext = c
+                    ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n_, Lc, ext)
+                    if grad is not None:
+                        ops().lpc_ragged_backward_(
+                            g, a, zi, y, table, dx, da, dzi, K, G, maxp, n_, Lc, True, True, True, ext
+                        )
+
+                step()
+                torch.cuda.synchronize()
+                times = []
+                for _ in range(reps):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    step()
+                    e1.record()
+                    torch.cuda.synchronize()
+                    times.append(e0.elapsed_time(e1))
+                ms[cand[5]] = sorted(times)[len(times) // 2]
+                if best is None or ms[cand[5]] < ms[best[5]]:
+                    best = cand
+            self.groups[gi] = best
+            out[n] = {"choice": best[5], "ms": ms}
+        return out
 
     def run(self, fn) -> None:
         """Calls ``fn(group)`` for every order group, on the plan's side streams
