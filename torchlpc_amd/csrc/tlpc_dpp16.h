@@ -134,6 +134,20 @@ __device__ __forceinline__ double chain_link(double s, const double (&m)[NP], do
   return (acc0 + acc1) + (acc2 + acc3);
 }
 
+// v of lane l ^ 16 (the other 16-lane row of the lane's 32-lane half), for a
+// lane whose row parity is h = (lane >> 4) & 1, without the LDS crossbar:
+// v_permlane16_swap exchanges the odd rows of its first operand with the even
+// rows of its second, so with v in both, the first result holds row 0's values
+// in row 1 (and row 2's in row 3) and the second holds row 1's in row 0.
+__device__ __forceinline__ double xor16(double v, int h) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+  const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const unsigned ol = h ? rl[0] : rl[1], oh = h ? rh[0] : rh[1];
+  return __builtin_bit_cast(double, ((unsigned long long)oh << 32) | ol);
+}
+
 // One link of a 32-state chain inside a 32-lane half-wave, without LDS: lane
 // (row half h = (lane >> 4) & 1, position l = lane & 15) holds state element
 // e = 16h + l in `t` and the columns 16h..16h+15 of matrix rows l (`ma`) and
@@ -164,7 +178,7 @@ __device__ __forceinline__ double dpp_matvec32(double t, const double (&ma)[16],
   const double pa = a0 + a1, pb = b0 + b1;
   // element l (h = 0) takes the upper half's partial of row l; element 16 + l
   // (h = 1) the lower half's partial of row l + 16
-  const double other = __shfl_xor(h ? pa : pb, 16);
+  const double other = xor16(h ? pa : pb, h);
   return (h ? pb : pa) + other;
 }
 
