@@ -2041,6 +2041,10 @@ bool ti_vec(const ChunkArgs<S>& a, const void* in, const void* out) {
 // streams (x or g, y, A, and the outputs that are present).
 template <typename S, int NP>
 constexpr bool kTv4 = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8);
+// The reruns with NP / 4 lanes per chunk (tlpc_grp_rerun.hip) exist for float32
+// orders 8, 16 and 32.
+template <typename S, int NP>
+constexpr bool kGrp = std::is_same<S, float>::value && (NP == 8 || NP == 16 || NP == 32);
 template <typename S>
 bool tv4_ok(const ChunkArgs<S>& a, int np, const void* y, const void* dx, const void* dA) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
@@ -2098,6 +2102,9 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
       hipLaunchKernelGGL((fwd_rerun_tv4<NP>), g, dim3(rt), 0, st, a);
       return hipGetLastError() == hipSuccess ? 0 : -4;
     }
+  }
+  if constexpr (kGrp<S, NP>) {
+    if (grp_rerun_ok(a, NP)) return launch_grp_fwd_rerun(a, NP, st);
   }
   hipLaunchKernelGGL((fwd_rerun<S, NP>), g, dim3(rt), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -4;
@@ -2183,7 +2190,15 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
       tv4 = true;
     }
   }
-  if (!tv4) hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(rt), 0, st, a);
+  bool grp = false;
+  if constexpr (kGrp<S, NP>) {
+    if (!tv4 && grp_rerun_ok(a, NP)) {
+      const int rc = launch_grp_bwd_rerun(a, NP, st);
+      if (rc != 0) return rc;
+      grp = true;
+    }
+  }
+  if (!tv4 && !grp) hipLaunchKernelGGL((bwd_rerun<S, NP>), g, dim3(rt), 0, st, a);
   if (a.dapart) {
     const int64_t nd = a.B * a.n;
     hipLaunchKernelGGL((ti_da_reduce<S, NP>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, a);

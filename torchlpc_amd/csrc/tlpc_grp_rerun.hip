@@ -1,0 +1,388 @@
+// Reruns (phase 3 of the chunked scan) with several lanes per chunk, for
+// float32 time-varying filters of order 8, 16 and 32 in uniform or ragged
+// batches: G = NP / 4 lanes share one chunk's serial recurrence, lane gl owning
+// lags 4gl+1 .. 4gl+4, so the G lanes of a chunk load a coefficient row as G
+// float4 (one 16 * G-byte piece) and a wave's row loads cover 64 / G rows
+// instead of 64. This is the quad scheme of the order-16 kernels (tlpc_dpp16.h)
+// with the group width a template parameter: partial sums meet through DPP
+// (quad_perm within quads, row_half_mirror across the two quads of an 8-lane
+// group), and the state windows move between neighbouring lanes through
+// row_shr:1 / row_shl:1.
+//
+// Arithmetic is float64 on the float32 data, as in the one-lane reruns, so the
+// only single-precision rounding of an output is its store. Rows, samples and
+// (adjoint) the entering y values are prefetched RING steps ahead at clamped
+// times; steps past the end of a signal compute on clamped data and store what
+// the last live step stored (forward) or have lambda = 0 (adjoint).
+//
+// Reference recurrence: torchlpc/csrc/scan_cpu.cpp:112-120; adjoint:
+// torchlpc/core.py:186-229.
+#include "tlpc_dpp16.h"
+#include "tlpc_kernels.h"
+
+namespace tlpc {
+namespace grp {
+
+using dpp16::qperm;
+
+constexpr int RING = 8;             // steps in flight per lane (a multiple of 4)
+constexpr int DPP_SHL1 = 0x101;     // row_shl:1  lane l reads lane l+1 of its 16-lane row
+constexpr int DPP_SHR1 = 0x111;     // row_shr:1  lane l reads lane l-1
+constexpr int DPP_HMIRROR = 0x141;  // row_half_mirror  lane l reads lane 7-l of its 8-lane half
+constexpr int QP_PAIR0 = 0xA0;      // quad_perm [0,0,2,2]
+
+// sum over the G lanes of a group, in every lane of it
+template <int G>
+__device__ __forceinline__ double gsum(double p) {
+  p += qperm<dpp16::QP_SWAP1>(p);
+  if constexpr (G >= 4) p += qperm<dpp16::QP_SWAP2>(p);
+  // after the quad sums, lane 7-l of an 8-lane half holds the other quad's sum
+  if constexpr (G >= 8) p += qperm<DPP_HMIRROR>(p);
+  return p;
+}
+
+// lane 0's value of a group, in every lane of it
+template <int G>
+__device__ __forceinline__ double gbcast0(double v, int gl) {
+  if constexpr (G == 2) {
+    return qperm<QP_PAIR0>(v);
+  } else if constexpr (G == 4) {
+    return qperm<dpp16::QP_LANE0>(v);
+  } else {
+    const double q = qperm<dpp16::QP_LANE0>(v);  // lane 0 of each quad
+    const double m = qperm<DPP_HMIRROR>(q);      // lanes 4-7 read lanes 3-0
+    return gl >= 4 ? m : q;
+  }
+}
+
+template <bool AL>
+__device__ __forceinline__ float4 ldq(const float* p) {
+  if constexpr (AL) {
+    return *reinterpret_cast<const float4*>(p);
+  } else {
+    return make_float4(p[0], p[1], p[2], p[3]);
+  }
+}
+template <bool AL>
+__device__ __forceinline__ void stq(float* p, float4 v) {
+  if constexpr (AL) {
+    *reinterpret_cast<float4*>(p) = v;
+  } else {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+    p[3] = v.w;
+  }
+}
+
+// Keeps a step's memory instructions where the source puts them (see pin_loads
+// in the order-16 kernels), and makes ring fills opaque so that a ring slot stays
+// a prefetch (a phi of two loads would be folded into one load at its use).
+__device__ __forceinline__ void pin() { __builtin_amdgcn_sched_barrier(0); }
+__device__ __forceinline__ float opq(float v) {
+  asm("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ float4 opq(float4 v) {
+  asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+  return v;
+}
+template <int K>
+__device__ __forceinline__ void settle(double (&v)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) asm volatile("" : "+v"(v[i]));
+}
+
+// y_u of one signal for the adjoint's window (y_{-1-k} = zi[k] for k < n, zero
+// past the order and at or past T): an always-valid address, and whether its
+// value counts.
+__device__ __forceinline__ const float* yw_addr(const float* yb, const float* zib, int n, int64_t T, int64_t u) {
+  const int64_t uc = u < T ? u : T - 1;
+  const int64_t k = -u - 1;
+  const float* zq = zib ? zib + (k < n ? k : n - 1) : yb;
+  return u >= 0 ? yb + uc : zq;
+}
+__device__ __forceinline__ bool yw_ok(const float* zib, int n, int64_t T, int64_t u) {
+  return u >= 0 ? u < T : (zib != nullptr && -u - 1 < n);
+}
+
+// ---------------------------------------------------------------------------
+// Forward. Lane gl's window R holds y_u for u in [t-4-4gl, t-1-4gl] at slot
+// u mod 4 (t0 is a multiple of 4, so step JJ of a ring pass has slot J = JJ & 3
+// for y_t); c is lane gl's float4 of row t (lags 4gl+1 .. 4gl+4).
+template <int J, int G>
+__device__ __forceinline__ double fwd_step(double (&R)[4], const float4 c, float x, int gl) {
+  // lag 4gl+1+m meets y_{t-1-4gl-m} in slot (J-1-m) & 3; the newest term goes last
+  double p = (double)c.w * R[(J - 4) & 3];
+  p = fma((double)c.z, R[(J - 3) & 3], p);
+  p = fma((double)c.y, R[(J - 2) & 3], p);
+  p = fma((double)c.x, R[(J - 1) & 3], p);
+  const double y = (double)x - gsum<G>(p);
+  // slot J (y_{t-4-4gl}) takes y_{t-4gl} from lane gl-1, or y_t in lane 0
+  const double up = qperm<DPP_SHR1>(R[J]);
+  R[J] = gl == 0 ? y : up;
+  return y;
+}
+
+template <int NP, bool AL, int JJ>
+__device__ __forceinline__ void fwd_ring(double (&R)[4], float4 (&cq)[RING], float (&xq)[RING], float& ylast,
+                                         const float* Ab, const float* xb, float* yb, int64_t T, int64_t tb,
+                                         int gl) {
+  if constexpr (JJ < RING) {
+    constexpr int G = NP / 4;
+    const int64_t t = tb + JJ;
+    const double y = fwd_step<JJ & 3, G>(R, cq[JJ], xq[JJ], gl);
+    const int64_t tn = min(t + RING, T - 1);
+    cq[JJ] = ldq<AL>(Ab + tn * NP + 4 * gl);
+    xq[JJ] = xb[tn];
+    pin();
+    // every lane of the group stores the same value; a step past the end stores
+    // the last live output at T-1 again
+    const bool live = t < T;
+    ylast = live ? (float)y : ylast;
+    yb[live ? t : T - 1] = ylast;
+    fwd_ring<NP, AL, JJ + 1>(R, cq, xq, ylast, Ab, xb, yb, T, tb, gl);
+  }
+}
+
+template <int NP, bool AL>
+__device__ __forceinline__ void fwd_chunk(const float* Ab, const float* xb, float* yb, const double* cr,
+                                          int64_t T, int64_t t0, int len, int gl) {
+  double R[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) R[3 - m] = cr[4 * gl + m];  // y_{t0-1-4gl-m}, slot 3-m
+  float4 cq[RING];
+  float xq[RING];
+#pragma unroll
+  for (int j = 0; j < RING; ++j) {
+    const int64_t tc = min(t0 + j, T - 1);
+    cq[j] = opq(ldq<AL>(Ab + tc * NP + 4 * gl));
+    xq[j] = opq(xb[tc]);
+  }
+  settle(R);
+  float ylast = 0.f;
+  for (int jb = 0; jb < len; jb += RING) fwd_ring<NP, AL, 0>(R, cq, xq, ylast, Ab, xb, yb, T, t0 + jb, gl);
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
+  constexpr int G = NP / 4;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t cid = gt / G;
+  const int gl = (int)(gt - cid * G);
+  if (cid >= (a.rg ? a.G : a.B * a.P)) return;  // a group's lanes leave together
+  const int64_t b = chunk_signal(a, cid);
+  const Span sp = sig_span(a, b);
+  const int64_t t0 = (cid - sp.cbase) * a.L;
+  const int64_t rem = sp.T - t0;
+  const int len = rem < a.L ? (int)rem : a.L;
+  const float* Ab = a.A + sp.ao;
+  const double* cr = a.carry + cid * NP;
+  if ((reinterpret_cast<uintptr_t>(Ab) & 15) == 0)
+    fwd_chunk<NP, true>(Ab, a.x + sp.xo, a.y + sp.xo, cr, sp.T, t0, len, gl);
+  else
+    fwd_chunk<NP, false>(Ab, a.x + sp.xo, a.y + sp.xo, cr, sp.T, t0, len, gl);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint, descending. `now` in lane 0 is the pending sum of position t; lane
+// gl's window P holds the pending sums of positions [t-4-4gl, t-1-4gl] at slot
+// u mod 4, and (with DA) its window Y the y values of the same positions.
+struct AdjStreams {
+  const float* Ab;
+  const float* gb;
+  const float* yb;
+  const float* zib;
+  float* dxb;
+  float* dAb;
+  float* sink;  // the group's consumed carry slot: destination of the stores of steps past T
+  int64_t T;
+  int n;
+};
+
+template <int NP, bool AL, bool DA, int JJ>
+__device__ __forceinline__ void adj_ring(double (&P)[4], double& now, float (&Y)[4], float4 (&cq)[RING],
+                                         float (&gq)[RING], float (&yq)[RING], const AdjStreams& io,
+                                         int64_t tb, int gl) {
+  if constexpr (JJ < RING) {
+    constexpr int G = NP / 4;
+    constexpr int J = (3 - JJ) & 3;  // slot of t = tb - JJ (tb = 3 mod 4)
+    const int64_t t = tb - JJ;
+    const bool live = t < io.T;
+    const double lam0 = gbcast0<G>((double)gq[JJ] + now, gl);
+    const double lam = live ? lam0 : 0.0;
+    const float4 c = cq[JJ];
+    P[(J - 1) & 3] = fma(-(double)c.x, lam, P[(J - 1) & 3]);
+    P[(J - 2) & 3] = fma(-(double)c.y, lam, P[(J - 2) & 3]);
+    P[(J - 3) & 3] = fma(-(double)c.z, lam, P[(J - 3) & 3]);
+    P[(J - 4) & 3] = fma(-(double)c.w, lam, P[(J - 4) & 3]);
+    // slot S (position t-1-4gl) leaves at the top: lane 0's becomes `now`, the
+    // others take their upper neighbour's, the last lane a fresh zero
+    constexpr int S = (J - 1) & 3;
+    const double down = qperm<DPP_SHL1>(P[S]);
+    now = P[S];
+    P[S] = gl == G - 1 ? 0.0 : down;
+    const float ynew = yq[JJ];
+    // refill the ring for step t - RING (times clamped into the signal; the y
+    // value entering lane G-1's window at that step is y_{t-RING-1-NP})
+    {
+      const int64_t tn = max(min(t - RING, io.T - 1), (int64_t)0);
+      cq[JJ] = ldq<AL>(io.Ab + tn * NP + 4 * gl);
+      gq[JJ] = io.gb[tn];
+      if constexpr (DA) {
+        const int64_t u = t - RING - 1 - NP;
+        const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
+        yq[JJ] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
+      }
+    }
+    pin();
+    const float lf = (float)lam;
+    if (io.dxb) *(live ? io.dxb + t : io.sink) = lf;
+    if constexpr (DA) {
+      float4 d;
+      d.x = -lf * Y[(J - 1) & 3];
+      d.y = -lf * Y[(J - 2) & 3];
+      d.z = -lf * Y[(J - 3) & 3];
+      d.w = -lf * Y[(J - 4) & 3];
+      stq<AL>(live ? io.dAb + t * NP + 4 * gl : io.sink + 4 * gl, d);
+      // the window moves down one position: slot S takes y_{t-5-4gl} from the lane
+      // above, the last lane y_{t-1-NP}
+      const float ydown = qperm<DPP_SHL1>(Y[S]);
+      Y[S] = gl == G - 1 ? ynew : ydown;
+    }
+    adj_ring<NP, AL, DA, JJ + 1>(P, now, Y, cq, gq, yq, io, tb, gl);
+  }
+}
+
+template <int NP, bool AL, bool DA>
+__device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr, int64_t t0, int L, int gl,
+                                          float* dz) {
+  double P[4];
+  double now = cr[0];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int rr = 4 * gl + 1 + m;
+    P[(2 - m) & 3] = rr < NP ? cr[rr] : 0.0;  // pending sum of position tstart-rr
+  }
+  const int64_t tstart = t0 + L - 1;
+  float Y[4] = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (DA) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int64_t u = tstart - 1 - 4 * gl - m;
+      const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
+      Y[(2 - m) & 3] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
+    }
+  }
+  float4 cq[RING];
+  float gq[RING], yq[RING];
+#pragma unroll
+  for (int j = 0; j < RING; ++j) {
+    const int64_t tc = max(min(tstart - j, io.T - 1), (int64_t)0);
+    cq[j] = opq(ldq<AL>(io.Ab + tc * NP + 4 * gl));
+    gq[j] = opq(io.gb[tc]);
+    yq[j] = 0.f;
+    if constexpr (DA) {
+      const int64_t u = tstart - j - 1 - NP;
+      const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
+      yq[j] = opq(yw_ok(io.zib, io.n, io.T, u) ? v : 0.f);
+    }
+  }
+  settle(P);
+  for (int jb = 0; jb < L; jb += RING) adj_ring<NP, AL, DA, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl);
+  if (dz) {
+    // pending sums of positions -1-r: r = 0 in `now` (lane 0), 4gl+1+m in slot (2-m)&3
+    if (gl == 0) dz[0] = (float)now;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int rr = 4 * gl + 1 + m;
+      if (rr < io.n) dz[rr] = (float)P[(2 - m) & 3];
+    }
+  }
+}
+
+template <int NP, bool DA>
+__global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
+  constexpr int G = NP / 4;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t cid = gt / G;
+  const int gl = (int)(gt - cid * G);
+  if (cid >= (a.rg ? a.G : a.B * a.P)) return;  // a group's lanes leave together
+  const int64_t b = chunk_signal(a, cid);
+  const Span sp = sig_span(a, b);
+  const int64_t c = cid - sp.cbase;
+  double* cr = a.carry + cid * NP;
+  AdjStreams io;
+  io.Ab = a.A + sp.ao;
+  io.gb = a.x + sp.xo;  // upstream gradient
+  io.yb = a.y + sp.xo;
+  io.zib = a.zi ? a.zi + sp.zo : nullptr;
+  io.dxb = a.dx ? a.dx + sp.xo : nullptr;
+  io.dAb = DA ? a.dA + sp.dao : nullptr;
+  // the carry was read before any store: NP doubles hold the group's 4G floats
+  io.sink = reinterpret_cast<float*>(cr);
+  io.T = sp.T;
+  io.n = a.n;
+  float* dz = (c == 0 && a.dzi) ? a.dzi + sp.zo : nullptr;
+  const bool al = ((reinterpret_cast<uintptr_t>(io.Ab) | reinterpret_cast<uintptr_t>(io.dAb)) & 15) == 0;
+  if (al)
+    adj_chunk<NP, true, DA>(io, cr, c * a.L, a.L, gl, dz);
+  else
+    adj_chunk<NP, false, DA>(io, cr, c * a.L, a.L, gl, dz);
+}
+
+inline dim3 grp_grid(const ChunkArgs<float>& a, int G, unsigned threads) {
+  const int64_t lanes = (a.rg ? a.G : a.B * a.P) * G;
+  return dim3((unsigned)((lanes + threads - 1) / threads));
+}
+// 64-lane workgroups under 64K lanes, so that few chunks still spread over the CUs
+inline unsigned grp_threads(const ChunkArgs<float>& a, int G) {
+  return (a.rg ? a.G : a.B * a.P) * G < 256 * 256 ? 64u : 256u;
+}
+
+}  // namespace grp
+
+bool grp_rerun_ok(const ChunkArgs<float>& a, int np) {
+  return (np == 8 || np == 16 || np == 32) && a.n == np && !a.neg && a.dapart == nullptr && a.rx == nullptr &&
+         (a.rg != nullptr || a.a_tstride == a.n) && a.L % grp::RING == 0 && a.L % 4 == 0;
+}
+
+int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
+  const int G = np / 4;
+  const unsigned th = grp::grp_threads(a, G);
+  const dim3 g = grp::grp_grid(a, G, th);
+  if (g.x == 0) return 0;
+  switch (np) {
+    case 8: hipLaunchKernelGGL(grp::grp_fwd_rerun<8>, g, dim3(th), 0, st, a); break;
+    case 16: hipLaunchKernelGGL(grp::grp_fwd_rerun<16>, g, dim3(th), 0, st, a); break;
+    case 32: hipLaunchKernelGGL(grp::grp_fwd_rerun<32>, g, dim3(th), 0, st, a); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
+  const int G = np / 4;
+  const unsigned th = grp::grp_threads(a, G);
+  const dim3 g = grp::grp_grid(a, G, th);
+  if (g.x == 0) return 0;
+  const bool da = a.dA != nullptr;
+  switch (np) {
+    case 8:
+      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<8, true>), g, dim3(th), 0, st, a);
+      else hipLaunchKernelGGL((grp::grp_bwd_rerun<8, false>), g, dim3(th), 0, st, a);
+      break;
+    case 16:
+      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<16, true>), g, dim3(th), 0, st, a);
+      else hipLaunchKernelGGL((grp::grp_bwd_rerun<16, false>), g, dim3(th), 0, st, a);
+      break;
+    case 32:
+      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<32, true>), g, dim3(th), 0, st, a);
+      else hipLaunchKernelGGL((grp::grp_bwd_rerun<32, false>), g, dim3(th), 0, st, a);
+      break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+}  // namespace tlpc

@@ -138,6 +138,13 @@ int launch_chunk16_carry(ChunkArgs<float> a, bool fwd, hipStream_t st);
 int launch_chunk16_fwd_rerun(ChunkArgs<float> a, hipStream_t st);
 int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st);
 
+// Reruns with NP / 4 lanes per chunk (float32 time-varying filters of order 8,
+// 16 and 32 whose rows are exactly NP wide, uniform or ragged; not the adjoint
+// that rebuilds y, nor time-invariant da partials).
+bool grp_rerun_ok(const ChunkArgs<float>& a, int np);
+int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
+int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
+
 // Per-signal tiled kernels (order 16, float32, time-varying): one workgroup per
 // signal, no workspace. a.y may be null in the backward when dA is not wanted.
 int launch_tile16_fwd(ChunkArgs<float> a, hipStream_t st);
