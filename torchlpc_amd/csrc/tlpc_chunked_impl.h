@@ -2045,6 +2045,10 @@ constexpr bool kTv4 = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP
 // orders 8, 16 and 32.
 template <typename S, int NP>
 constexpr bool kGrp = std::is_same<S, float>::value && (NP == 8 || NP == 16 || NP == 32);
+// The summaries on DPP row broadcasts (tlpc_row_summary.hip) exist for float32
+// orders 16 and 32.
+template <typename S, int NP>
+constexpr bool kRowSum = std::is_same<S, float>::value && (NP == 16 || NP == 32);
 template <typename S>
 bool tv4_ok(const ChunkArgs<S>& a, int np, const void* y, const void* dx, const void* dA) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
@@ -2081,9 +2085,19 @@ int launch_chunked_fwd(ChunkArgs<S> a, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   if (a.P > 1) {
-    using CF = ChunkCfg<S, NP, NP>;
-    const int64_t grid = (summary_items(a) + CF::G - 1) / CF::G;
-    hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(CF::THREADS), 0, st, a);
+    bool done = false;
+    if constexpr (kRowSum<S, NP>) {
+      if (row_summary_ok(a, NP)) {
+        const int rc = launch_row_summary(a, NP, true, st);
+        if (rc != 0) return rc;
+        done = true;
+      }
+    }
+    if (!done) {
+      using CF = ChunkCfg<S, NP, NP>;
+      const int64_t grid = (summary_items(a) + CF::G - 1) / CF::G;
+      hipLaunchKernelGGL((fwd_summary<S, NP>), dim3((unsigned)grid), dim3(CF::THREADS), 0, st, a);
+    }
   }
   bool blocked = false;
   if constexpr (blocked_fits<S, NP>()) {
@@ -2151,7 +2165,15 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     }
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
-  if (a.P > 1) {
+  bool rowsum = false;
+  if constexpr (kRowSum<S, NP>) {
+    if (a.P > 1 && row_summary_ok(a, NP)) {
+      const int rc = launch_row_summary(a, NP, false, st);
+      if (rc != 0) return rc;
+      rowsum = true;
+    }
+  }
+  if (a.P > 1 && !rowsum) {
     if constexpr (NP == 32) {
       using CB = ChunkCfg<S, NP, NP>;
       const int64_t grid = (summary_items(a) + CB::G - 1) / CB::G;

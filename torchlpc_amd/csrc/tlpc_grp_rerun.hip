@@ -19,11 +19,13 @@
 // torchlpc/core.py:186-229.
 #include "tlpc_dpp16.h"
 #include "tlpc_kernels.h"
+#include "tlpc_vmem.h"
 
 namespace tlpc {
 namespace grp {
 
 using dpp16::qperm;
+using namespace vmem;
 
 constexpr int RING = 8;             // steps in flight per lane (a multiple of 4)
 constexpr int DPP_SHL1 = 0x101;     // row_shl:1  lane l reads lane l+1 of its 16-lane row
@@ -53,44 +55,6 @@ __device__ __forceinline__ double gbcast0(double v, int gl) {
     const double m = qperm<DPP_HMIRROR>(q);      // lanes 4-7 read lanes 3-0
     return gl >= 4 ? m : q;
   }
-}
-
-template <bool AL>
-__device__ __forceinline__ float4 ldq(const float* p) {
-  if constexpr (AL) {
-    return *reinterpret_cast<const float4*>(p);
-  } else {
-    return make_float4(p[0], p[1], p[2], p[3]);
-  }
-}
-template <bool AL>
-__device__ __forceinline__ void stq(float* p, float4 v) {
-  if constexpr (AL) {
-    *reinterpret_cast<float4*>(p) = v;
-  } else {
-    p[0] = v.x;
-    p[1] = v.y;
-    p[2] = v.z;
-    p[3] = v.w;
-  }
-}
-
-// Keeps a step's memory instructions where the source puts them (see pin_loads
-// in the order-16 kernels), and makes ring fills opaque so that a ring slot stays
-// a prefetch (a phi of two loads would be folded into one load at its use).
-__device__ __forceinline__ void pin() { __builtin_amdgcn_sched_barrier(0); }
-__device__ __forceinline__ float opq(float v) {
-  asm("" : "+v"(v));
-  return v;
-}
-__device__ __forceinline__ float4 opq(float4 v) {
-  asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
-  return v;
-}
-template <int K>
-__device__ __forceinline__ void settle(double (&v)[K]) {
-#pragma unroll
-  for (int i = 0; i < K; ++i) asm volatile("" : "+v"(v[i]));
 }
 
 // y_u of one signal for the adjoint's window (y_{-1-k} = zi[k] for k < n, zero

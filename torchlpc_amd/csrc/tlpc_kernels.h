@@ -144,6 +144,10 @@ int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st);
 bool grp_rerun_ok(const ChunkArgs<float>& a, int np);
 int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
 int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
+// Chunk summaries on DPP row broadcasts (tlpc_row_summary.hip): float32
+// time-varying orders 16 and 32 with n == NP, plain rows, L a multiple of NP.
+bool row_summary_ok(const ChunkArgs<float>& a, int np);
+int launch_row_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st);
 
 // Per-signal tiled kernels (order 16, float32, time-varying): one workgroup per
 // signal, no workspace. a.y may be null in the backward when dA is not wanted.
