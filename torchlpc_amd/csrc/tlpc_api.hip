@@ -53,14 +53,21 @@ inline size_t elem_size(int dtype) {
   }
 }
 
-// Chunk length: a multiple of the staging block, sized so that the rerun phase
-// has about 32K lanes in flight, capped at 4096 samples, and at least 4*NP long
-// so that the carry chains stay short. Summaries and carries are in the wide
-// type (float64 for single-precision data), so a chunk boundary adds no
-// single-precision rounding and shorter chunks cost no accuracy.
-inline int pick_chunk(int64_t B, int64_t T, int np, int requested) {
+// Chunk length: a multiple of the staging block, at least 4*NP and at most 4096
+// samples, sized for about 32K rerun lanes: one lane per chunk, or NP/4 for
+// float32 orders 8, 16 and 32 that fill their padded width (n == NP; the group
+// reruns take whole coefficient rows), whose chunks are then NP/4 times longer
+// and their carry chains that much shorter. In the ragged workload's order
+// groups (about 3.3M samples each) the fastest lengths were 224 / 416 / 864
+// samples at orders 8 / 16 / 32 (0.32 / 0.39 / 0.81 ms per step), about 32K
+// lanes each, against 0.42 / 0.65 / 1.22 ms with 32K chunks. Summaries and carries are in the wide type (float64 for
+// single-precision data), so a chunk boundary adds no single-precision rounding
+// and shorter chunks cost no accuracy.
+inline bool multi_lane_rerun(int dtype, int n, int np) { return dtype == TLPC_F32 && np >= 8 && n == np; }
+inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool multi_lane) {
   const int blk = chunk_block(np);
-  int64_t L = requested > 0 ? requested : (B * T + 32767) / 32768;
+  const int64_t target = multi_lane ? 32768 / (np / 4) : 32768;  // chunks
+  int64_t L = requested > 0 ? requested : (B * T + target - 1) / target;
   if (requested <= 0 && L < 4 * np) L = 4 * np;
   if (L < blk) L = blk;
   if (L > 4096 && requested <= 0) L = 4096;
@@ -93,7 +100,7 @@ inline int device_cus() {
 // are chosen first, P = 64 * CUs / B, and the length rounded up to 16 samples
 // from there (which can only lower P). At least 16 * 16 samples.
 inline int pick_chunk16(int64_t B, int64_t T, int requested) {
-  if (requested > 0) return pick_chunk(B, T, 16, requested);
+  if (requested > 0) return pick_chunk(B, T, 16, requested, true);
   int64_t P = 64 * (int64_t)device_cus() / (B > 0 ? B : 1);
   if (P < 1) P = 1;
   int64_t L = (T + P - 1) / P;
@@ -214,7 +221,7 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
   else if (chunk16_ok(dtype, n, flags, neg, 256, T))
     p.L = pick_chunk16(B, T, chunk_len);
   else
-    p.L = pick_chunk(B, T, p.np, chunk_len);
+    p.L = pick_chunk(B, T, p.np, chunk_len, multi_lane_rerun(dtype, n, p.np));
   p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
   const size_t ws_es = wide_size(dtype);
   // time-invariant summaries are the zero-state vectors only (Phi is per signal)
@@ -645,7 +652,8 @@ TLPC_EXPORT int tlpc_lpc_ragged_plan(int64_t K, const int64_t* x_off, const int6
     total += lengths[k];
   }
   if (elem_size(dtype) == 0) return TLPC_EDTYPE;
-  const int L = pick_chunk(K > 0 ? K : 1, K > 0 ? (total + K - 1) / K : 1, np, chunk_len);
+  const int L = pick_chunk(K > 0 ? K : 1, K > 0 ? (total + K - 1) / K : 1, np, chunk_len,
+                           multi_lane_rerun(dtype, N, np));
   int64_t G = 0, maxP = 0;
   for (int64_t k = 0; k < K; ++k) {
     const int64_t P = (lengths[k] + L - 1) / L;

@@ -1,13 +1,14 @@
-// Reruns (phase 3 of the chunked scan) with several lanes per chunk, for
-// float32 time-varying filters of order 8, 16 and 32 in uniform or ragged
-// batches: G = NP / 4 lanes share one chunk's serial recurrence, lane gl owning
-// lags 4gl+1 .. 4gl+4, so the G lanes of a chunk load a coefficient row as G
-// float4 (one 16 * G-byte piece) and a wave's row loads cover 64 / G rows
-// instead of 64. This is the quad scheme of the order-16 kernels (tlpc_dpp16.h)
-// with the group width a template parameter: partial sums meet through DPP
-// (quad_perm within quads, row_half_mirror across the two quads of an 8-lane
-// group), and the state windows move between neighbouring lanes through
-// row_shr:1 / row_shl:1.
+// Reruns (phase 3 of the chunked scan) with prefetched streams and, from order
+// 8 on, several lanes per chunk, for float32 time-varying filters of order 2,
+// 4, 8, 16 and 32 in uniform or ragged batches: G = NP / 4 lanes (one at orders
+// 2 and 4) share one chunk's serial recurrence, lane gl owning lags 4gl+1 ..
+// 4gl+4, so the G lanes of a chunk load a coefficient row as G float4 (one
+// 16 * G-byte piece; an order-2 row is one float2) and a wave's row loads cover
+// 64 / G rows instead of 64. This is the quad scheme of the order-16 kernels
+// (tlpc_dpp16.h) with the group width a template parameter: partial sums meet
+// through DPP (quad_perm within quads, row_half_mirror across the two quads of
+// an 8-lane group), and the state windows move between neighbouring lanes
+// through row_shr:1 / row_shl:1.
 //
 // Arithmetic is float64 on the float32 data, as in the one-lane reruns, so the
 // only single-precision rounding of an output is its store. Rows, samples and
@@ -27,7 +28,17 @@ namespace grp {
 using dpp16::qperm;
 using namespace vmem;
 
-constexpr int RING = 8;             // steps in flight per lane (a multiple of 4)
+// Group shape: G lanes per chunk, RW floats of a row per lane (an order-2 row
+// is padded to four coefficients in registers, the other two zero), RING steps
+// in flight per lane (a multiple of 4; deeper for the one-lane groups, whose
+// steps are short), and WIN = 4G positions in the windows of a group.
+template <int NP>
+struct Grp {
+  static constexpr int G = NP >= 4 ? NP / 4 : 1;
+  static constexpr int RW = NP >= 4 ? 4 : NP;
+  static constexpr int RING = NP <= 4 ? 16 : 8;
+  static constexpr int WIN = 4 * G;
+};
 constexpr int DPP_SHL1 = 0x101;     // row_shl:1  lane l reads lane l+1 of its 16-lane row
 constexpr int DPP_SHR1 = 0x111;     // row_shr:1  lane l reads lane l-1
 constexpr int DPP_HMIRROR = 0x141;  // row_half_mirror  lane l reads lane 7-l of its 8-lane half
@@ -36,6 +47,7 @@ constexpr int QP_PAIR0 = 0xA0;      // quad_perm [0,0,2,2]
 // sum over the G lanes of a group, in every lane of it
 template <int G>
 __device__ __forceinline__ double gsum(double p) {
+  if constexpr (G == 1) return p;
   p += qperm<dpp16::QP_SWAP1>(p);
   if constexpr (G >= 4) p += qperm<dpp16::QP_SWAP2>(p);
   // after the quad sums, lane 7-l of an 8-lane half holds the other quad's sum
@@ -46,7 +58,9 @@ __device__ __forceinline__ double gsum(double p) {
 // lane 0's value of a group, in every lane of it
 template <int G>
 __device__ __forceinline__ double gbcast0(double v, int gl) {
-  if constexpr (G == 2) {
+  if constexpr (G == 1) {
+    return v;
+  } else if constexpr (G == 2) {
     return qperm<QP_PAIR0>(v);
   } else if constexpr (G == 4) {
     return qperm<dpp16::QP_LANE0>(v);
@@ -55,6 +69,35 @@ __device__ __forceinline__ double gbcast0(double v, int gl) {
     const double m = qperm<DPP_HMIRROR>(q);      // lanes 4-7 read lanes 3-0
     return gl >= 4 ? m : q;
   }
+}
+
+// Lane's piece of a coefficient row (RW floats, zero-padded to four) and of a
+// dA row; AL: the piece is aligned to its size.
+template <int NP, bool AL>
+__device__ __forceinline__ float4 ldrow(const float* p) {
+  if constexpr (Grp<NP>::RW == 4) {
+    return ldq<AL>(p);
+  } else if constexpr (AL) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    return make_float4(v.x, v.y, 0.f, 0.f);
+  } else {
+    return make_float4(p[0], p[1], 0.f, 0.f);
+  }
+}
+template <int NP, bool AL>
+__device__ __forceinline__ void strow(float* p, float4 v) {
+  if constexpr (Grp<NP>::RW == 4) {
+    stq<AL>(p, v);
+  } else if constexpr (AL) {
+    *reinterpret_cast<float2*>(p) = make_float2(v.x, v.y);
+  } else {
+    p[0] = v.x;
+    p[1] = v.y;
+  }
+}
+template <int NP>
+__device__ __forceinline__ bool row_aligned(const void* p) {
+  return (reinterpret_cast<uintptr_t>(p) & (4 * Grp<NP>::RW - 1)) == 0;
 }
 
 // y_u of one signal for the adjoint's window (y_{-1-k} = zi[k] for k < n, zero
@@ -89,15 +132,16 @@ __device__ __forceinline__ double fwd_step(double (&R)[4], const float4 c, float
 }
 
 template <int NP, bool AL, int JJ>
-__device__ __forceinline__ void fwd_ring(double (&R)[4], float4 (&cq)[RING], float (&xq)[RING], float& ylast,
-                                         const float* Ab, const float* xb, float* yb, int64_t T, int64_t tb,
-                                         int gl) {
+__device__ __forceinline__ void fwd_ring(double (&R)[4], float4 (&cq)[Grp<NP>::RING], float (&xq)[Grp<NP>::RING],
+                                         float& ylast, const float* Ab, const float* xb, float* yb, int64_t T,
+                                         int64_t tb, int gl) {
+  constexpr int RING = Grp<NP>::RING;
   if constexpr (JJ < RING) {
-    constexpr int G = NP / 4;
+    constexpr int G = Grp<NP>::G;
     const int64_t t = tb + JJ;
     const double y = fwd_step<JJ & 3, G>(R, cq[JJ], xq[JJ], gl);
     const int64_t tn = min(t + RING, T - 1);
-    cq[JJ] = ldq<AL>(Ab + tn * NP + 4 * gl);
+    cq[JJ] = ldrow<NP, AL>(Ab + tn * NP + 4 * gl);
     xq[JJ] = xb[tn];
     pin();
     // every lane of the group stores the same value; a step past the end stores
@@ -112,15 +156,17 @@ __device__ __forceinline__ void fwd_ring(double (&R)[4], float4 (&cq)[RING], flo
 template <int NP, bool AL>
 __device__ __forceinline__ void fwd_chunk(const float* Ab, const float* xb, float* yb, const double* cr,
                                           int64_t T, int64_t t0, int len, int gl) {
+  constexpr int RING = Grp<NP>::RING;
   double R[4];
+  // y_{t0-1-4gl-m} in slot 3-m (lags past an order-2 row meet zero coefficients)
 #pragma unroll
-  for (int m = 0; m < 4; ++m) R[3 - m] = cr[4 * gl + m];  // y_{t0-1-4gl-m}, slot 3-m
+  for (int m = 0; m < 4; ++m) R[3 - m] = m < NP ? cr[4 * gl + m] : 0.0;
   float4 cq[RING];
   float xq[RING];
 #pragma unroll
   for (int j = 0; j < RING; ++j) {
     const int64_t tc = min(t0 + j, T - 1);
-    cq[j] = opq(ldq<AL>(Ab + tc * NP + 4 * gl));
+    cq[j] = opq(ldrow<NP, AL>(Ab + tc * NP + 4 * gl));
     xq[j] = opq(xb[tc]);
   }
   settle(R);
@@ -130,7 +176,7 @@ __device__ __forceinline__ void fwd_chunk(const float* Ab, const float* xb, floa
 
 template <int NP>
 __global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
-  constexpr int G = NP / 4;
+  constexpr int G = Grp<NP>::G;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t cid = gt / G;
   const int gl = (int)(gt - cid * G);
@@ -142,7 +188,7 @@ __global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
   const int len = rem < a.L ? (int)rem : a.L;
   const float* Ab = a.A + sp.ao;
   const double* cr = a.carry + cid * NP;
-  if ((reinterpret_cast<uintptr_t>(Ab) & 15) == 0)
+  if (row_aligned<NP>(Ab))
     fwd_chunk<NP, true>(Ab, a.x + sp.xo, a.y + sp.xo, cr, sp.T, t0, len, gl);
   else
     fwd_chunk<NP, false>(Ab, a.x + sp.xo, a.y + sp.xo, cr, sp.T, t0, len, gl);
@@ -165,11 +211,12 @@ struct AdjStreams {
 };
 
 template <int NP, bool AL, bool DA, int JJ>
-__device__ __forceinline__ void adj_ring(double (&P)[4], double& now, float (&Y)[4], float4 (&cq)[RING],
-                                         float (&gq)[RING], float (&yq)[RING], const AdjStreams& io,
-                                         int64_t tb, int gl) {
+__device__ __forceinline__ void adj_ring(double (&P)[4], double& now, float (&Y)[4],
+                                         float4 (&cq)[Grp<NP>::RING], float (&gq)[Grp<NP>::RING],
+                                         float (&yq)[Grp<NP>::RING], const AdjStreams& io, int64_t tb, int gl) {
+  constexpr int RING = Grp<NP>::RING;
   if constexpr (JJ < RING) {
-    constexpr int G = NP / 4;
+    constexpr int G = Grp<NP>::G;
     constexpr int J = (3 - JJ) & 3;  // slot of t = tb - JJ (tb = 3 mod 4)
     const int64_t t = tb - JJ;
     const bool live = t < io.T;
@@ -188,13 +235,13 @@ __device__ __forceinline__ void adj_ring(double (&P)[4], double& now, float (&Y)
     P[S] = gl == G - 1 ? 0.0 : down;
     const float ynew = yq[JJ];
     // refill the ring for step t - RING (times clamped into the signal; the y
-    // value entering lane G-1's window at that step is y_{t-RING-1-NP})
+    // value entering lane G-1's window at that step is y_{t-RING-1-WIN})
     {
       const int64_t tn = max(min(t - RING, io.T - 1), (int64_t)0);
-      cq[JJ] = ldq<AL>(io.Ab + tn * NP + 4 * gl);
+      cq[JJ] = ldrow<NP, AL>(io.Ab + tn * NP + 4 * gl);
       gq[JJ] = io.gb[tn];
       if constexpr (DA) {
-        const int64_t u = t - RING - 1 - NP;
+        const int64_t u = t - RING - 1 - Grp<NP>::WIN;
         const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
         yq[JJ] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
       }
@@ -208,9 +255,9 @@ __device__ __forceinline__ void adj_ring(double (&P)[4], double& now, float (&Y)
       d.y = -lf * Y[(J - 2) & 3];
       d.z = -lf * Y[(J - 3) & 3];
       d.w = -lf * Y[(J - 4) & 3];
-      stq<AL>(live ? io.dAb + t * NP + 4 * gl : io.sink + 4 * gl, d);
+      strow<NP, AL>(live ? io.dAb + t * NP + 4 * gl : io.sink + 4 * gl, d);
       // the window moves down one position: slot S takes y_{t-5-4gl} from the lane
-      // above, the last lane y_{t-1-NP}
+      // above, the last lane y_{t-1-WIN}
       const float ydown = qperm<DPP_SHL1>(Y[S]);
       Y[S] = gl == G - 1 ? ynew : ydown;
     }
@@ -238,16 +285,17 @@ __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr
       Y[(2 - m) & 3] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
     }
   }
+  constexpr int RING = Grp<NP>::RING;
   float4 cq[RING];
   float gq[RING], yq[RING];
 #pragma unroll
   for (int j = 0; j < RING; ++j) {
     const int64_t tc = max(min(tstart - j, io.T - 1), (int64_t)0);
-    cq[j] = opq(ldq<AL>(io.Ab + tc * NP + 4 * gl));
+    cq[j] = opq(ldrow<NP, AL>(io.Ab + tc * NP + 4 * gl));
     gq[j] = opq(io.gb[tc]);
     yq[j] = 0.f;
     if constexpr (DA) {
-      const int64_t u = tstart - j - 1 - NP;
+      const int64_t u = tstart - j - 1 - Grp<NP>::WIN;
       const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
       yq[j] = opq(yw_ok(io.zib, io.n, io.T, u) ? v : 0.f);
     }
@@ -267,7 +315,7 @@ __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr
 
 template <int NP, bool DA>
 __global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
-  constexpr int G = NP / 4;
+  constexpr int G = Grp<NP>::G;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t cid = gt / G;
   const int gl = (int)(gt - cid * G);
@@ -283,12 +331,12 @@ __global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
   io.zib = a.zi ? a.zi + sp.zo : nullptr;
   io.dxb = a.dx ? a.dx + sp.xo : nullptr;
   io.dAb = DA ? a.dA + sp.dao : nullptr;
-  // the carry was read before any store: NP doubles hold the group's 4G floats
+  // the carry was read before any store: NP doubles hold the group's G * RW floats
   io.sink = reinterpret_cast<float*>(cr);
   io.T = sp.T;
   io.n = a.n;
   float* dz = (c == 0 && a.dzi) ? a.dzi + sp.zo : nullptr;
-  const bool al = ((reinterpret_cast<uintptr_t>(io.Ab) | reinterpret_cast<uintptr_t>(io.dAb)) & 15) == 0;
+  const bool al = row_aligned<NP>(io.Ab) && (!DA || row_aligned<NP>(io.dAb));
   if (al)
     adj_chunk<NP, true, DA>(io, cr, c * a.L, a.L, gl, dz);
   else
@@ -299,24 +347,27 @@ inline dim3 grp_grid(const ChunkArgs<float>& a, int G, unsigned threads) {
   const int64_t lanes = (a.rg ? a.G : a.B * a.P) * G;
   return dim3((unsigned)((lanes + threads - 1) / threads));
 }
-// 64-lane workgroups under 64K lanes, so that few chunks still spread over the CUs
-inline unsigned grp_threads(const ChunkArgs<float>& a, int G) {
-  return (a.rg ? a.G : a.B * a.P) * G < 256 * 256 ? 64u : 256u;
-}
+// 64-lane workgroups: the launches hold some 32K lanes, and 256-lane ones would
+// leave a few hundred workgroups to balance over 256 CUs (order 32 in the ragged
+// workload, 384-sample chunks: 1.11 ms per step against 0.88 ms at 448 samples,
+// the first length under the former 64K-lane switch to 64-lane workgroups)
+inline unsigned grp_threads(const ChunkArgs<float>&, int) { return 64u; }
 
 }  // namespace grp
 
 bool grp_rerun_ok(const ChunkArgs<float>& a, int np) {
-  return (np == 8 || np == 16 || np == 32) && a.n == np && !a.neg && a.dapart == nullptr && a.rx == nullptr &&
-         (a.rg != nullptr || a.a_tstride == a.n) && a.L % grp::RING == 0 && a.L % 4 == 0;
+  return (np == 2 || np == 4 || np == 8 || np == 16 || np == 32) && a.n == np && !a.neg && a.dapart == nullptr &&
+         a.rx == nullptr && (a.rg != nullptr || a.a_tstride == a.n) && a.L % 16 == 0;
 }
 
 int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
-  const int G = np / 4;
+  const int G = np >= 4 ? np / 4 : 1;
   const unsigned th = grp::grp_threads(a, G);
   const dim3 g = grp::grp_grid(a, G, th);
   if (g.x == 0) return 0;
   switch (np) {
+    case 2: hipLaunchKernelGGL(grp::grp_fwd_rerun<2>, g, dim3(th), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(grp::grp_fwd_rerun<4>, g, dim3(th), 0, st, a); break;
     case 8: hipLaunchKernelGGL(grp::grp_fwd_rerun<8>, g, dim3(th), 0, st, a); break;
     case 16: hipLaunchKernelGGL(grp::grp_fwd_rerun<16>, g, dim3(th), 0, st, a); break;
     case 32: hipLaunchKernelGGL(grp::grp_fwd_rerun<32>, g, dim3(th), 0, st, a); break;
@@ -326,12 +377,20 @@ int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
 }
 
 int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
-  const int G = np / 4;
+  const int G = np >= 4 ? np / 4 : 1;
   const unsigned th = grp::grp_threads(a, G);
   const dim3 g = grp::grp_grid(a, G, th);
   if (g.x == 0) return 0;
   const bool da = a.dA != nullptr;
   switch (np) {
+    case 2:
+      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<2, true>), g, dim3(th), 0, st, a);
+      else hipLaunchKernelGGL((grp::grp_bwd_rerun<2, false>), g, dim3(th), 0, st, a);
+      break;
+    case 4:
+      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<4, true>), g, dim3(th), 0, st, a);
+      else hipLaunchKernelGGL((grp::grp_bwd_rerun<4, false>), g, dim3(th), 0, st, a);
+      break;
     case 8:
       if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<8, true>), g, dim3(th), 0, st, a);
       else hipLaunchKernelGGL((grp::grp_bwd_rerun<8, false>), g, dim3(th), 0, st, a);

@@ -29,7 +29,9 @@ namespace rsum {
 using namespace vmem;
 using dpp16::fma_bcast;
 
-constexpr int THREADS = 256;
+// 64-lane workgroups (two chunks at order 32, four at order 16): a launch of a
+// few thousand chunks then spreads evenly over the CUs.
+constexpr int THREADS = 64;
 
 // Where coefficient k of row jr of a 4-row group sits: lane L0(jr) + (k >> 2) of
 // the 16-lane row, component k & 3 of the lane's float4 number D(jr).
