@@ -347,7 +347,7 @@ def bench_workload(name, steps, warmup, ragged_order=0, tune=True):
                         fwd(grp)
                         bwd(grp)
                 return
-            # the order groups run on the plan's side streams (graph branches)
+            # the order groups, one after another on the current stream
             plan.run(fwd)
             plan.run(bwd)
 

@@ -80,7 +80,10 @@ def test_ragged_mixed_lengths_and_orders(dtype):
     _check(sigs, dtype, plan, y, X.grad, AA.grad, Z.grad, REL_TOL[dtype])
 
 
-def test_ragged_full_workload_and_graph_replay():
+@pytest.mark.parametrize("streams", [1, 4])
+def test_ragged_full_workload_and_graph_replay(streams):
+    """The order groups in order on one stream (the default), or on side streams
+    (parallel branches of the captured graph)."""
     from torchlpc_amd import RaggedPlan
     from torchlpc_amd.ragged import RaggedLPC
 
@@ -89,7 +92,8 @@ def test_ragged_full_workload_and_graph_replay():
     orders = [s[1].shape[1] for s in sigs]
     x, a, zi, g = _pack(sigs, np.float32)
     X, AA, Z, Gd = dev(x), dev(a), dev(zi), dev(g)
-    plan = RaggedPlan(lengths, orders, X.device, X.dtype)
+    plan = RaggedPlan(lengths, orders, X.device, X.dtype, streams=streams)
+    assert max(plan.lanes) == min(streams, 5) - 1
     ops = torch.ops.torchlpc_amd
 
     def step():
@@ -100,7 +104,7 @@ def test_ragged_full_workload_and_graph_replay():
             n, table, K, G, maxp, L, ext = grp
             ops.lpc_ragged_backward_(Gd, AA, Z, y, table, dx, da, dzi, K, G, maxp, n, L, True, True, True, ext)
 
-        plan.run(bwd)  # order groups on the plan's side streams
+        plan.run(bwd)
         return y, dx, da, dzi
 
     eager = step()
