@@ -1,14 +1,15 @@
-// Reruns (phase 3 of the chunked scan) with prefetched streams and, from order
-// 8 on, several lanes per chunk, for float32 time-varying filters of order 2,
-// 4, 8, 16 and 32 in uniform or ragged batches: G = NP / 4 lanes (one at orders
-// 2 and 4) share one chunk's serial recurrence, lane gl owning lags 4gl+1 ..
-// 4gl+4, so the G lanes of a chunk load a coefficient row as G float4 (one
-// 16 * G-byte piece; an order-2 row is one float2) and a wave's row loads cover
-// 64 / G rows instead of 64. This is the quad scheme of the order-16 kernels
-// (tlpc_dpp16.h) with the group width a template parameter: partial sums meet
-// through DPP (quad_perm within quads, row_half_mirror across the two quads of
-// an 8-lane group), and the state windows move between neighbouring lanes
-// through row_shr:1 / row_shl:1.
+// Reruns (phase 3 of the chunked scan) with prefetched streams and several
+// lanes per chunk, for float32 time-varying filters of order 2, 4, 8, 16 and 32
+// in uniform or ragged batches. Forward: G = NP / 4 lanes (one at orders 2 and
+// 4) share one chunk's serial recurrence, lane gl owning lags 4gl+1 .. 4gl+4,
+// so the G lanes of a chunk load a coefficient row as G float4 (one 16 * G-byte
+// piece; an order-2 row is one float2) and a wave's row loads cover 64 / G rows
+// instead of 64. Adjoint: NP / 2 lanes with two lags each. This is the quad
+// scheme of the order-16 kernels (tlpc_dpp16.h) with the group width a template
+// parameter: partial sums meet through DPP (quad_perm within quads,
+// row_half_mirror across the two quads of an 8-lane group), lambda leaves the
+// group's lane 0 the same way, and the state windows move between neighbouring
+// lanes through row_shr:1 / row_shl:1.
 //
 // Arithmetic is float64 on the float32 data, as in the one-lane reruns, so the
 // only single-precision rounding of an output is its store. Rows, samples and
@@ -28,16 +29,15 @@ namespace grp {
 using dpp16::qperm;
 using namespace vmem;
 
-// Group shape: G lanes per chunk, RW floats of a row per lane (an order-2 row
-// is padded to four coefficients in registers, the other two zero), RING steps
-// in flight per lane (a multiple of 4; deeper for the one-lane groups, whose
-// steps are short), and WIN = 4G positions in the windows of a group.
+// Forward group shape: G lanes per chunk, RW floats of a row per lane (an
+// order-2 row is padded to four coefficients in registers, the other two zero),
+// RING steps in flight per lane (a multiple of 4; deeper for the one-lane
+// groups, whose steps are short).
 template <int NP>
 struct Grp {
   static constexpr int G = NP >= 4 ? NP / 4 : 1;
   static constexpr int RW = NP >= 4 ? 4 : NP;
   static constexpr int RING = NP <= 4 ? 16 : 8;
-  static constexpr int WIN = 4 * G;
 };
 constexpr int DPP_SHL1 = 0x101;     // row_shl:1  lane l reads lane l+1 of its 16-lane row
 constexpr int DPP_SHR1 = 0x111;     // row_shr:1  lane l reads lane l-1
@@ -71,8 +71,8 @@ __device__ __forceinline__ double gbcast0(double v, int gl) {
   }
 }
 
-// Lane's piece of a coefficient row (RW floats, zero-padded to four) and of a
-// dA row; AL: the piece is aligned to its size.
+// Lane's piece of a coefficient row (RW floats, zero-padded to four); AL: the
+// piece is aligned to its size.
 template <int NP, bool AL>
 __device__ __forceinline__ float4 ldrow(const float* p) {
   if constexpr (Grp<NP>::RW == 4) {
@@ -82,17 +82,6 @@ __device__ __forceinline__ float4 ldrow(const float* p) {
     return make_float4(v.x, v.y, 0.f, 0.f);
   } else {
     return make_float4(p[0], p[1], 0.f, 0.f);
-  }
-}
-template <int NP, bool AL>
-__device__ __forceinline__ void strow(float* p, float4 v) {
-  if constexpr (Grp<NP>::RW == 4) {
-    stq<AL>(p, v);
-  } else if constexpr (AL) {
-    *reinterpret_cast<float2*>(p) = make_float2(v.x, v.y);
-  } else {
-    p[0] = v.x;
-    p[1] = v.y;
   }
 }
 template <int NP>
@@ -111,6 +100,29 @@ __device__ __forceinline__ const float* yw_addr(const float* yb, const float* zi
 }
 __device__ __forceinline__ bool yw_ok(const float* zib, int n, int64_t T, int64_t u) {
   return u >= 0 ? u < T : (zib != nullptr && -u - 1 < n);
+}
+// The same two, without a branch, for the adjoint's step loop: a branch there
+// lets the wait-count pass drain the prefetch ring at its join, and so does a
+// select on a value right after its load, so the ring keeps the raw value and
+// the step that consumes it applies the select. zb is the signal's zi, or any
+// valid address when there is none (haszi false).
+// Both candidate offsets pass through an empty asm statement before the select
+// (on offsets, not pointers, which would lose their global address space), and
+// the flag is formed bitwise: otherwise the compiler turns the conditionals into
+// control flow.
+__device__ __forceinline__ const float* yw_ptr(const float* yb, const float* zb, int n, int64_t T, int64_t u) {
+  const int64_t k = -u - 1;
+  int64_t op = u < T ? u : T - 1;
+  int64_t on = k < n ? k : n - 1;
+  asm("" : "+v"(op), "+v"(on));
+  const bool pos = u >= 0;
+  return (pos ? yb : zb) + (pos ? op : on);
+}
+__device__ __forceinline__ bool yw_flag(bool haszi, int n, int64_t T, int64_t u) {
+  const int pos = u >= 0;
+  int ok = (pos & (int)(u < T)) | ((pos ^ 1) & (int)haszi & (int)(-u - 1 < n));
+  asm("" : "+v"(ok));
+  return ok != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -195,14 +207,52 @@ __global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint, descending. `now` in lane 0 is the pending sum of position t; lane
-// gl's window P holds the pending sums of positions [t-4-4gl, t-1-4gl] at slot
-// u mod 4, and (with DA) its window Y the y values of the same positions.
+// Adjoint, descending, with two lags per lane: G = NP / 2 lanes per chunk (one
+// at order 2), lane gl owning lags 2gl+1 and 2gl+2, so a group loads a row as G
+// float2 and stores its dA row the same way. The adjoint needs no group sum,
+// only lambda broadcast from the group's lane 0, so twice the forward's lanes
+// per chunk cost one DPP move per step and give the latency-bound steps twice
+// the waves. `now` in lane 0 is the pending sum of position t; lane gl's window
+// P holds the pending sums of positions [t-2-2gl, t-1-2gl] at slot u mod 2, and
+// (with DA) its window Y the y values of the same positions.
+template <int NP>
+struct AdjGrp {
+  static constexpr int G = NP / 2;
+  static constexpr int RING = NP <= 4 ? 16 : 8;  // steps in flight (the chunk length is a multiple)
+};
+
+template <int G>
+__device__ __forceinline__ double gbcast0_adj(double v, int gl) {
+  if constexpr (G == 16) {
+    double r = 0.0;
+    dpp16::fma_bcast<0>(r, v, 1.0);  // lane 0 of the 16-lane row, times one
+    return r;
+  } else {
+    return gbcast0<G>(v, gl);
+  }
+}
+
+template <bool AL>
+__device__ __forceinline__ float2 ld2(const float* p) {
+  if constexpr (AL) return *reinterpret_cast<const float2*>(p);
+  return make_float2(p[0], p[1]);
+}
+template <bool AL>
+__device__ __forceinline__ void st2(float* p, float2 v) {
+  if constexpr (AL) {
+    *reinterpret_cast<float2*>(p) = v;
+  } else {
+    p[0] = v.x;
+    p[1] = v.y;
+  }
+}
+
 struct AdjStreams {
   const float* Ab;
   const float* gb;
   const float* yb;
   const float* zib;
+  const float* zb;  // zib, or yb when there is no zi (an always-valid address)
   float* dxb;
   float* dAb;
   float* sink;  // the group's consumed carry slot: destination of the stores of steps past T
@@ -210,112 +260,104 @@ struct AdjStreams {
   int n;
 };
 
-template <int NP, bool AL, bool DA, int JJ>
-__device__ __forceinline__ void adj_ring(double (&P)[4], double& now, float (&Y)[4],
-                                         float4 (&cq)[Grp<NP>::RING], float (&gq)[Grp<NP>::RING],
-                                         float (&yq)[Grp<NP>::RING], const AdjStreams& io, int64_t tb, int gl) {
-  constexpr int RING = Grp<NP>::RING;
+// DX / DA: whether dx / dA are written (template arguments, so that the step
+// loop holds no branch).
+template <int NP, bool AL, bool DA, bool DX, int JJ>
+__device__ __forceinline__ void adj_ring(double (&P)[2], double& now, float (&Y)[2],
+                                         float2 (&cq)[AdjGrp<NP>::RING], float (&gq)[AdjGrp<NP>::RING],
+                                         float (&yq)[AdjGrp<NP>::RING], const AdjStreams& io, int64_t tb,
+                                         int gl) {
+  constexpr int RING = AdjGrp<NP>::RING;
   if constexpr (JJ < RING) {
-    constexpr int G = Grp<NP>::G;
-    constexpr int J = (3 - JJ) & 3;  // slot of t = tb - JJ (tb = 3 mod 4)
+    constexpr int G = AdjGrp<NP>::G;
+    constexpr int J = (1 - JJ) & 1;  // slot of t = tb - JJ (tb odd)
     const int64_t t = tb - JJ;
     const bool live = t < io.T;
-    const double lam0 = gbcast0<G>((double)gq[JJ] + now, gl);
+    const double lam0 = gbcast0_adj<G>((double)gq[JJ] + now, gl);
     const double lam = live ? lam0 : 0.0;
-    const float4 c = cq[JJ];
-    P[(J - 1) & 3] = fma(-(double)c.x, lam, P[(J - 1) & 3]);
-    P[(J - 2) & 3] = fma(-(double)c.y, lam, P[(J - 2) & 3]);
-    P[(J - 3) & 3] = fma(-(double)c.z, lam, P[(J - 3) & 3]);
-    P[(J - 4) & 3] = fma(-(double)c.w, lam, P[(J - 4) & 3]);
-    // slot S (position t-1-4gl) leaves at the top: lane 0's becomes `now`, the
+    const float2 c = cq[JJ];
+    P[(J - 1) & 1] = fma(-(double)c.x, lam, P[(J - 1) & 1]);
+    P[J] = fma(-(double)c.y, lam, P[J]);
+    // slot S (position t-1-2gl) leaves at the top: lane 0's becomes `now`, the
     // others take their upper neighbour's, the last lane a fresh zero
-    constexpr int S = (J - 1) & 3;
+    constexpr int S = (J - 1) & 1;
     const double down = qperm<DPP_SHL1>(P[S]);
     now = P[S];
     P[S] = gl == G - 1 ? 0.0 : down;
-    const float ynew = yq[JJ];
+    const float ynew = yw_flag(io.zib != nullptr, io.n, io.T, t - 1 - NP) ? yq[JJ] : 0.f;
     // refill the ring for step t - RING (times clamped into the signal; the y
-    // value entering lane G-1's window at that step is y_{t-RING-1-WIN})
+    // value entering lane G-1's window at that step is y_{t-RING-1-NP})
     {
       const int64_t tn = max(min(t - RING, io.T - 1), (int64_t)0);
-      cq[JJ] = ldrow<NP, AL>(io.Ab + tn * NP + 4 * gl);
+      cq[JJ] = ld2<AL>(io.Ab + tn * NP + 2 * gl);
       gq[JJ] = io.gb[tn];
-      if constexpr (DA) {
-        const int64_t u = t - RING - 1 - Grp<NP>::WIN;
-        const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
-        yq[JJ] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
-      }
+      if constexpr (DA) yq[JJ] = *yw_ptr(io.yb, io.zb, io.n, io.T, t - RING - 1 - NP);
     }
     pin();
     const float lf = (float)lam;
-    if (io.dxb) *(live ? io.dxb + t : io.sink) = lf;
+    if constexpr (DX) *(live ? io.dxb + t : io.sink) = lf;
     if constexpr (DA) {
-      float4 d;
-      d.x = -lf * Y[(J - 1) & 3];
-      d.y = -lf * Y[(J - 2) & 3];
-      d.z = -lf * Y[(J - 3) & 3];
-      d.w = -lf * Y[(J - 4) & 3];
-      strow<NP, AL>(live ? io.dAb + t * NP + 4 * gl : io.sink + 4 * gl, d);
-      // the window moves down one position: slot S takes y_{t-5-4gl} from the lane
-      // above, the last lane y_{t-1-WIN}
+      const float2 d = make_float2(-lf * Y[(J - 1) & 1], -lf * Y[J]);
+      st2<AL>(live ? io.dAb + t * NP + 2 * gl : io.sink + 2 * gl, d);
+      // the window moves down one position: slot S takes y_{t-3-2gl} from the lane
+      // above, the last lane y_{t-1-NP}
       const float ydown = qperm<DPP_SHL1>(Y[S]);
       Y[S] = gl == G - 1 ? ynew : ydown;
     }
-    adj_ring<NP, AL, DA, JJ + 1>(P, now, Y, cq, gq, yq, io, tb, gl);
+    adj_ring<NP, AL, DA, DX, JJ + 1>(P, now, Y, cq, gq, yq, io, tb, gl);
   }
 }
 
-template <int NP, bool AL, bool DA>
+template <int NP, bool AL, bool DA, bool DX>
 __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr, int64_t t0, int L, int gl,
                                           float* dz) {
-  double P[4];
+  // tstart = t0 + L - 1 is odd: position tstart - (2gl+1+m) sits in slot m & 1
+  // (m = 0, 1), in P as in Y
+  double P[2];
   double now = cr[0];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int rr = 4 * gl + 1 + m;
-    P[(2 - m) & 3] = rr < NP ? cr[rr] : 0.0;  // pending sum of position tstart-rr
+  for (int m = 0; m < 2; ++m) {
+    const int rr = 2 * gl + 1 + m;
+    P[m] = rr < NP ? cr[rr] : 0.0;  // pending sum of position tstart-rr
   }
   const int64_t tstart = t0 + L - 1;
-  float Y[4] = {0.f, 0.f, 0.f, 0.f};
+  float Y[2] = {0.f, 0.f};
   if constexpr (DA) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int64_t u = tstart - 1 - 4 * gl - m;
+    for (int m = 0; m < 2; ++m) {
+      const int64_t u = tstart - 1 - 2 * gl - m;
       const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
-      Y[(2 - m) & 3] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
+      Y[m] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
     }
   }
-  constexpr int RING = Grp<NP>::RING;
-  float4 cq[RING];
+  constexpr int RING = AdjGrp<NP>::RING;
+  float2 cq[RING];
   float gq[RING], yq[RING];
 #pragma unroll
   for (int j = 0; j < RING; ++j) {
     const int64_t tc = max(min(tstart - j, io.T - 1), (int64_t)0);
-    cq[j] = opq(ldrow<NP, AL>(io.Ab + tc * NP + 4 * gl));
+    cq[j] = opq(ld2<AL>(io.Ab + tc * NP + 2 * gl));
     gq[j] = opq(io.gb[tc]);
     yq[j] = 0.f;
-    if constexpr (DA) {
-      const int64_t u = tstart - j - 1 - Grp<NP>::WIN;
-      const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
-      yq[j] = opq(yw_ok(io.zib, io.n, io.T, u) ? v : 0.f);
-    }
+    if constexpr (DA) yq[j] = opq(*yw_ptr(io.yb, io.zb, io.n, io.T, tstart - j - 1 - NP));
   }
   settle(P);
-  for (int jb = 0; jb < L; jb += RING) adj_ring<NP, AL, DA, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl);
+  for (int jb = 0; jb < L; jb += RING)
+    adj_ring<NP, AL, DA, DX, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl);
   if (dz) {
-    // pending sums of positions -1-r: r = 0 in `now` (lane 0), 4gl+1+m in slot (2-m)&3
+    // pending sums of positions -1-r: r = 0 in `now` (lane 0), 2gl+1+m in slot m
     if (gl == 0) dz[0] = (float)now;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int rr = 4 * gl + 1 + m;
-      if (rr < io.n) dz[rr] = (float)P[(2 - m) & 3];
+    for (int m = 0; m < 2; ++m) {
+      const int rr = 2 * gl + 1 + m;
+      if (rr < io.n) dz[rr] = (float)P[m];
     }
   }
 }
 
 template <int NP, bool DA>
 __global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
-  constexpr int G = Grp<NP>::G;
+  constexpr int G = AdjGrp<NP>::G;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t cid = gt / G;
   const int gl = (int)(gt - cid * G);
@@ -329,18 +371,26 @@ __global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
   io.gb = a.x + sp.xo;  // upstream gradient
   io.yb = a.y + sp.xo;
   io.zib = a.zi ? a.zi + sp.zo : nullptr;
+  io.zb = io.zib ? io.zib : io.yb;
   io.dxb = a.dx ? a.dx + sp.xo : nullptr;
   io.dAb = DA ? a.dA + sp.dao : nullptr;
-  // the carry was read before any store: NP doubles hold the group's G * RW floats
+  // the carry was read before any store: its NP doubles hold the group's 2G floats
   io.sink = reinterpret_cast<float*>(cr);
   io.T = sp.T;
   io.n = a.n;
   float* dz = (c == 0 && a.dzi) ? a.dzi + sp.zo : nullptr;
-  const bool al = row_aligned<NP>(io.Ab) && (!DA || row_aligned<NP>(io.dAb));
-  if (al)
-    adj_chunk<NP, true, DA>(io, cr, c * a.L, a.L, gl, dz);
-  else
-    adj_chunk<NP, false, DA>(io, cr, c * a.L, a.L, gl, dz);
+  const bool al = ((reinterpret_cast<uintptr_t>(io.Ab) | (DA ? reinterpret_cast<uintptr_t>(io.dAb) : 0)) & 7) == 0;
+  if (io.dxb) {
+    if (al)
+      adj_chunk<NP, true, DA, true>(io, cr, c * a.L, a.L, gl, dz);
+    else
+      adj_chunk<NP, false, DA, true>(io, cr, c * a.L, a.L, gl, dz);
+  } else {
+    if (al)
+      adj_chunk<NP, true, DA, false>(io, cr, c * a.L, a.L, gl, dz);
+    else
+      adj_chunk<NP, false, DA, false>(io, cr, c * a.L, a.L, gl, dz);
+  }
 }
 
 inline dim3 grp_grid(const ChunkArgs<float>& a, int G, unsigned threads) {
@@ -377,7 +427,7 @@ int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
 }
 
 int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
-  const int G = np >= 4 ? np / 4 : 1;
+  const int G = np / 2;
   const unsigned th = grp::grp_threads(a, G);
   const dim3 g = grp::grp_grid(a, G, th);
   if (g.x == 0) return 0;

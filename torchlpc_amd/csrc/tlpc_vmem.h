@@ -43,6 +43,10 @@ __device__ __forceinline__ float opq(float v) {
   asm("" : "+v"(v));
   return v;
 }
+__device__ __forceinline__ float2 opq(float2 v) {
+  asm("" : "+v"(v.x), "+v"(v.y));
+  return v;
+}
 __device__ __forceinline__ float4 opq(float4 v) {
   asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
   return v;
