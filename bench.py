@@ -329,27 +329,37 @@ def bench_workload(name, steps, warmup, ragged_order=0, tune=True):
         y = torch.empty_like(x)
         dx, dA, dzi = torch.empty_like(x), torch.empty_like(A), torch.empty_like(zi)
 
-        def fwd(grp):
-            n, table, Kg, G, maxp, L, ext = grp
-            ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L, ext)
+        from torchlpc_amd.ragged import keeps_summaries
 
-        def bwd(grp):
+        kept = {}
+
+        # as the autograd Function runs a training step: the float32 order-16
+        # and order-32 groups keep their forward's summaries for the backward
+        def fwd(gi, grp):
             n, table, Kg, G, maxp, L, ext = grp
-            ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True, ext)
+            if keeps_summaries(grp, f32):
+                kept[gi] = ops.lpc_ragged_keep_(x, A, zi, table, y, Kg, G, maxp, n, L, ext)
+            else:
+                ops.lpc_ragged_(x, A, zi, table, y, Kg, G, maxp, n, L, ext)
+
+        def bwd(gi, grp):
+            n, table, Kg, G, maxp, L, ext = grp
+            ops.lpc_ragged_backward_(w, A, zi, y, table, dx, dA, dzi, Kg, G, maxp, n, L, True, True, True, ext,
+                                     kept.pop(gi, None))
 
         if ragged_order and ragged_order not in orders:
             raise SystemExit(f"--ragged-order {ragged_order}: no signal of that order (orders are 2, 4, 8, 16, 32)")
 
         def step():
             if ragged_order:
-                for grp in plan.groups:
+                for gi, grp in enumerate(plan.groups):
                     if grp[0] == ragged_order:
-                        fwd(grp)
-                        bwd(grp)
+                        fwd(gi, grp)
+                        bwd(gi, grp)
                 return
             # the order groups, one after another on the current stream
-            plan.run(fwd)
-            plan.run(bwd)
+            plan.run_indexed(fwd)
+            plan.run_indexed(bwd)
 
         sel = [(T, N) for T, N in zip(lengths, orders) if not ragged_order or N == ragged_order]
         S = sum(T for T, _ in sel)

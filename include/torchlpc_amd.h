@@ -178,6 +178,14 @@ size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32_t N, int32_
  * query table_len, G and the chunk length first). The caller copies the table
  * to device memory and passes it, with K, G, max_chunks and the chunk length,
  * to the launches. Time-varying coefficients only.
+ *
+ * tlpc_lpc_ragged_backward takes TLPC_REUSE_SUMMARY in `flags` when its workspace
+ * still holds what tlpc_lpc_ragged_forward wrote there for the same table, A,
+ * N, dtype and chunk length: for float32 orders 16 and 32 the backward then
+ * reads the forward's chunk transition matrices (transposed) instead of forming
+ * its own, and overwrites only the rest of the workspace, so the flag may be
+ * given again to a later backward with the same workspace. It is ignored for
+ * other orders and dtypes (the backward then forms its own summaries there).
  */
 int tlpc_lpc_ragged_plan(int64_t K, const int64_t* x_off, const int64_t* a_off,
                          const int64_t* z_off, const int64_t* lengths, int32_t N, int32_t dtype,

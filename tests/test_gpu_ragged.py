@@ -182,3 +182,79 @@ def test_ragged_tune_per_group_chunk_lengths():
     y = ragged_lpc(Xg, Ag, plan, Zg)
     y.backward(Gd)
     _check(sigs, np.float32, plan, y, Xg.grad, Ag.grad, Zg.grad, REL_TOL[np.float32])
+
+
+def test_ragged_kept_summaries_backward():
+    """With gradients on, the float32 order-16 and order-32 groups keep their
+    forward's workspace and the backward reads the forward's chunk transition
+    matrices (transposed) instead of forming its own: the gradients match the
+    FP64 oracle and the backward that forms its own summaries, including chunks
+    that end past a signal's end, one-chunk signals, rows that are not 16-byte
+    aligned (an odd-length order-1 signal before them), a repeated backward on
+    the same workspace, and a HIP-graph replay of the kept step."""
+    from torchlpc_amd import RaggedPlan, ragged_lpc
+    from torchlpc_amd.ragged import keeps_summaries
+
+    rng = np.random.default_rng(29)
+    lengths = [5, 3001, 4096, 100, 2500, 6000, 777, 3333, 64, 9000]
+    orders = [1, 16, 32, 16, 32, 16, 32, 8, 32, 16]
+    sigs = _mixed(rng, lengths, orders)
+    x, a, zi, g = _pack(sigs, np.float32)
+    X, AA, Z, Gd = dev(x), dev(a), dev(zi), dev(g)
+    plan = RaggedPlan(lengths, orders, X.device, X.dtype, chunk_len=128)
+    assert any(keeps_summaries(grp, X.dtype) for grp in plan.groups)
+    ops = torch.ops.torchlpc_amd
+
+    # autograd: the kept path, checked against FP64
+    Xg, Ag, Zg = (t.clone().requires_grad_() for t in (X, AA, Z))
+    y = ragged_lpc(Xg, Ag, plan, Zg)
+    y.backward(Gd)
+    _check(sigs, np.float32, plan, y, Xg.grad, Ag.grad, Zg.grad, REL_TOL[np.float32])
+
+    # raw ops: kept against formed summaries, and the kept workspace used twice
+    def run(kept_mode, steps=1):
+        y = torch.empty_like(X)
+        ws = {}
+        for gi, grp in enumerate(plan.groups):
+            n, table, K, G, maxp, L, ext = grp
+            if kept_mode and keeps_summaries(grp, X.dtype):
+                ws[gi] = ops.lpc_ragged_keep_(X, AA, Z, table, y, K, G, maxp, n, L, ext)
+            else:
+                ops.lpc_ragged_(X, AA, Z, table, y, K, G, maxp, n, L, ext)
+        outs = []
+        for _ in range(steps):
+            dx, da, dzi = torch.empty_like(X), torch.empty_like(AA), torch.empty_like(Z)
+            for gi, grp in enumerate(plan.groups):
+                n, table, K, G, maxp, L, ext = grp
+                ops.lpc_ragged_backward_(Gd, AA, Z, y, table, dx, da, dzi, K, G, maxp, n, L, True, True, True,
+                                         ext, ws.get(gi))
+            outs.append((dx, da, dzi))
+        return y, outs
+
+    y0, [(dx0, da0, dz0)] = run(False)
+    y1, [(dx1, da1, dz1), (dx2, da2, dz2)] = run(True, steps=2)
+    assert torch.equal(y0, y1)
+    for u, v in ((dx0, dx1), (da0, da1), (dz0, dz1)):
+        assert rel(host(u), host(v)) <= 1e-5
+    for u, v in ((dx1, dx2), (da1, da2), (dz1, dz2)):
+        assert torch.equal(u, v)
+
+    # graph replay of the kept step
+    def step():
+        return run(True)[1][0]
+
+    eager = [t.clone() for t in step()]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = step()
+    for r in range(3):
+        torch.randn(1 << 20, device="cuda").mul_(3.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        for e, o in zip(eager, outs):
+            assert torch.equal(e, o), r

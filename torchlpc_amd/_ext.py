@@ -22,6 +22,7 @@ _OPS = (
     "lpc_backward_recompute",
     "lpc_ragged_plan",
     "lpc_ragged_",
+    "lpc_ragged_keep_",
     "lpc_ragged_backward_",
     "scan",
     "scan_backward",
@@ -110,6 +111,16 @@ def recompute_supported(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chu
     fn.argtypes = [ctypes.c_int, i64, i64, i32, i32, i32, i32]
     B, T = x.shape
     return fn(_DTYPE_CODE[x.dtype], B, T, A.shape[-1], 7, strategy, chunk_len) == 1
+
+
+def ragged_workspace_bytes(dtype: torch.dtype, G: int, N: int) -> int:
+    """``tlpc_lpc_ragged_workspace`` for one order group of a ragged plan (host only)."""
+    import ctypes
+
+    fn = _hiplib().tlpc_lpc_ragged_workspace
+    fn.restype = ctypes.c_size_t
+    fn.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int32]
+    return int(fn(_DTYPE_CODE[dtype], G, N))
 
 
 def forward_workspace_bytes(x: torch.Tensor, A: torch.Tensor, strategy: int = 0, chunk_len: int = 0) -> int:
@@ -208,8 +219,13 @@ def _register_fake() -> None:
     def _lpc_ragged_fake(x, A, zi, table, y, K, G, max_chunks, N, chunk_len, extents):
         return None
 
+    @torch.library.register_fake(lib + "lpc_ragged_keep_")
+    def _lpc_ragged_keep_fake(x, A, zi, table, y, K, G, max_chunks, N, chunk_len, extents):
+        return x.new_empty((ragged_workspace_bytes(x.dtype, G, N),), dtype=torch.uint8)
+
     @torch.library.register_fake(lib + "lpc_ragged_backward_")
-    def _lpc_ragged_bwd_fake(g, A, zi, y, table, dx, dA, dzi, K, G, max_chunks, N, chunk_len, nx, na, nz, extents):
+    def _lpc_ragged_bwd_fake(g, A, zi, y, table, dx, dA, dzi, K, G, max_chunks, N, chunk_len, nx, na, nz, extents,
+                             kept=None):
         return None
 
     @torch.library.register_fake(lib + "scan")

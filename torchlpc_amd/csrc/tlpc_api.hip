@@ -469,6 +469,7 @@ int run_ragged(const void* x, const void* A, const void* zi, void* y, void* dx, 
   a.a_tstride = n;
   a.rg = rg;
   a.G = G;
+  a.kept = backward && (flags & TLPC_REUSE_SUMMARY) ? 1 : 0;
   using W = typename Wide<S>::type;
   char* w = static_cast<char*>(ws);
   a.summ = reinterpret_cast<W*>(w);

@@ -421,13 +421,27 @@ __global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
       stage[q] = a.summ[summ_slot(a, b, sp, c, FWD) * C::SUMM + off];
     }
   };
+  // An adjoint handed the forward's summaries (a.kept) reads each chunk's Phi
+  // as Psi = Phi^T, transposing while it commits, and the matrix of its first
+  // link (the signal's last chunk, which has no forward summary) as zero: that
+  // link meets the zero state, and the slot holds whatever was there.
+  const bool tr = !FWD && a.kept;
   auto commit = [&](int64_t j) {
     W(*dst)[C::LINK] = buf[j & 1];
 #pragma unroll
     for (int q = 0; q < C::PER; ++q) {
       const int e = q * 64 + lane;
       const int li = e / C::SUMM, off = e - li * C::SUMM;
-      if (e < C::CB * C::SUMM) dst[li][off < NP * NP ? off + off / NP : off + NP] = stage[q];
+      if (e < C::CB * C::SUMM) {
+        if (off >= NP * NP) {
+          dst[li][off + NP] = stage[q];
+        } else if (!tr) {
+          dst[li][off + off / NP] = stage[q];
+        } else {
+          const int k = off / NP, r = off - k * NP;  // Phi[k][r] is Psi[r][k]
+          dst[li][r * C::ROWP + k] = (j * C::CB + li == 0) ? Num<W>::zero() : stage[q];
+        }
+      }
     }
   };
   const bool own = wave == 0 && lane < NP;
@@ -2166,13 +2180,20 @@ int launch_chunked_bwd(ChunkArgs<S> a, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -4;
   }
   bool rowsum = false;
+  bool kept = false;
   if constexpr (kRowSum<S, NP>) {
-    if (a.P > 1 && row_summary_ok(a, NP)) {
+    // with the forward's summaries at hand only the zero-state responses are new
+    if (a.P > 1 && a.kept && grp_wsum_ok(a, NP)) {
+      const int rc = launch_grp_wsum(a, NP, st);
+      if (rc != 0) return rc;
+      rowsum = kept = true;
+    } else if (a.P > 1 && row_summary_ok(a, NP)) {
       const int rc = launch_row_summary(a, NP, false, st);
       if (rc != 0) return rc;
       rowsum = true;
     }
   }
+  a.kept = kept ? 1 : 0;  // the chain reads the slots transposed only when they hold Phi
   if (a.P > 1 && !rowsum) {
     if constexpr (NP == 32) {
       using CB = ChunkCfg<S, NP, NP>;

@@ -393,6 +393,62 @@ __global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
   }
 }
 
+// Adjoint zero-state responses of chunks 1 .. P-1 for a backward that has the
+// forward's transition matrices (the adjoint carry chain reads them transposed):
+// the group's adjoint recurrence over the chunk from zero pending sums, driven
+// by g, leaves w[r], the pending sum of position t0-1-r, which is stored in the
+// chunk's summary slot after its matrix (where the forward kept z). N FMAs per
+// sample instead of the N^2 of a summary that also forms the matrix.
+template <int NP, bool AL>
+__device__ __forceinline__ void wsum_chunk(const AdjStreams& io, int64_t t0, int L, int gl, double* out) {
+  constexpr int RING = AdjGrp<NP>::RING;
+  double P[2] = {0.0, 0.0};
+  double now = 0.0;
+  float Y[2] = {0.f, 0.f};
+  const int64_t tstart = t0 + L - 1;
+  float2 cq[RING];
+  float gq[RING], yq[RING];
+#pragma unroll
+  for (int j = 0; j < RING; ++j) {
+    const int64_t tc = max(min(tstart - j, io.T - 1), (int64_t)0);
+    cq[j] = opq(ld2<AL>(io.Ab + tc * NP + 2 * gl));
+    gq[j] = opq(io.gb[tc]);
+    yq[j] = 0.f;
+  }
+  for (int jb = 0; jb < L; jb += RING)
+    adj_ring<NP, AL, false, false, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl);
+  // pending sums of positions t0-1-r: r = 0 in `now` (lane 0), 2gl+1+m in slot m
+  if (gl == 0) out[0] = now;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int rr = 2 * gl + 1 + m;
+    if (rr < NP) out[rr] = P[m];
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void grp_wsum(ChunkArgs<float> a) {
+  constexpr int G = AdjGrp<NP>::G;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t cid = gt / G;
+  const int gl = (int)(gt - cid * G);
+  if (cid >= a.G) return;  // a group's lanes leave together
+  const int64_t b = chunk_signal(a, cid);
+  const Span sp = sig_span(a, b);
+  const int64_t c = cid - sp.cbase;
+  if (c < 1) return;  // the chain stops at chunk 0's carry
+  AdjStreams io{};
+  io.Ab = a.A + sp.ao;
+  io.gb = a.x + sp.xo;  // upstream gradient
+  io.T = sp.T;
+  io.n = a.n;
+  double* out = a.summ + summ_slot(a, b, sp, c, false) * (NP * NP + NP) + NP * NP;
+  if ((reinterpret_cast<uintptr_t>(io.Ab) & 7) == 0)
+    wsum_chunk<NP, true>(io, c * a.L, a.L, gl, out);
+  else
+    wsum_chunk<NP, false>(io, c * a.L, a.L, gl, out);
+}
+
 inline dim3 grp_grid(const ChunkArgs<float>& a, int G, unsigned threads) {
   const int64_t lanes = (a.rg ? a.G : a.B * a.P) * G;
   return dim3((unsigned)((lanes + threads - 1) / threads));
@@ -421,6 +477,23 @@ int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
     case 8: hipLaunchKernelGGL(grp::grp_fwd_rerun<8>, g, dim3(th), 0, st, a); break;
     case 16: hipLaunchKernelGGL(grp::grp_fwd_rerun<16>, g, dim3(th), 0, st, a); break;
     case 32: hipLaunchKernelGGL(grp::grp_fwd_rerun<32>, g, dim3(th), 0, st, a); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+bool grp_wsum_ok(const ChunkArgs<float>& a, int np) {
+  return (np == 16 || np == 32) && a.rg != nullptr && grp_rerun_ok(a, np);
+}
+
+int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st) {
+  const int G = np / 2;
+  const unsigned th = grp::grp_threads(a, G);
+  const dim3 g = grp::grp_grid(a, G, th);
+  if (g.x == 0) return 0;
+  switch (np) {
+    case 16: hipLaunchKernelGGL(grp::grp_wsum<16>, g, dim3(th), 0, st, a); break;
+    case 32: hipLaunchKernelGGL(grp::grp_wsum<32>, g, dim3(th), 0, st, a); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -4;

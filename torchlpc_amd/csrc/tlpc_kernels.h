@@ -54,6 +54,10 @@ struct ChunkArgs {
   // chunk carries [B, P, NP], from which the adjoint rerun rebuilds y.
   const S* rx;
   const typename Wide<S>::type* fcarry;
+  // Adjoint of a ragged batch whose summary slots still hold the forward's
+  // transition matrices (slot of chunk c: Phi_c): the adjoint chain reads them
+  // transposed (Psi_c = Phi_c^T) and only the zero-state responses are formed.
+  int kept;
 };
 
 // Floats per signal of ChunkArgs::ypad: the y window of an order-NP adjoint
@@ -144,6 +148,10 @@ int launch_chunk16_adj_rerun(ChunkArgs<float> a, hipStream_t st);
 bool grp_rerun_ok(const ChunkArgs<float>& a, int np);
 int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
 int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
+// Adjoint zero-state responses only, for a ragged backward handed the forward's
+// summaries (ChunkArgs::kept): float32 orders 16 and 32.
+bool grp_wsum_ok(const ChunkArgs<float>& a, int np);
+int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st);
 // Chunk summaries on DPP row broadcasts (tlpc_row_summary.hip): float32
 // time-varying orders 16 and 32 with n == NP, plain rows, L a multiple of NP.
 bool row_summary_ok(const ChunkArgs<float>& a, int np);

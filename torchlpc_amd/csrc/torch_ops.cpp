@@ -352,12 +352,14 @@ void check_ragged(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi
               extents[1], ", ", extents[2], ")");
 }
 
-void lpc_ragged_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
-                    const at::Tensor& table, at::Tensor& y, int64_t K, int64_t G, int64_t maxP,
-                    int64_t N, int64_t chunk_len, at::IntArrayRef extents) {
-  check_ragged(x, A, zi, table, K, G, extents, "lpc_ragged_");
+// Ragged forward into y; returns the workspace, which holds the chunk summaries
+// a later lpc_ragged_backward_ may be handed (`kept`).
+at::Tensor ragged_forward(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
+                          const at::Tensor& table, at::Tensor& y, int64_t K, int64_t G, int64_t maxP,
+                          int64_t N, int64_t chunk_len, at::IntArrayRef extents, const char* op) {
+  check_ragged(x, A, zi, table, K, G, extents, op);
   TORCH_CHECK(y.sizes() == x.sizes() && y.scalar_type() == x.scalar_type() && y.is_contiguous(),
-              "torchlpc_amd::lpc_ragged_: y must be a contiguous tensor like x");
+              "torchlpc_amd::", op, ": y must be a contiguous tensor like x");
   c10::DeviceGuard guard(x.device());
   const int dt = dtype_code(x);
   const size_t wsb = tlpc_lpc_ragged_workspace(dt, G, (int32_t)N);
@@ -367,13 +369,27 @@ void lpc_ragged_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& 
                                    (int32_t)chunk_len, wsb ? ws.data_ptr() : nullptr, wsb,
                                    stream_of(x)),
            "ragged forward");
+  return ws;
+}
+
+void lpc_ragged_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
+                    const at::Tensor& table, at::Tensor& y, int64_t K, int64_t G, int64_t maxP,
+                    int64_t N, int64_t chunk_len, at::IntArrayRef extents) {
+  ragged_forward(x, A, zi, table, y, K, G, maxP, N, chunk_len, extents, "lpc_ragged_");
+}
+
+at::Tensor lpc_ragged_keep_hip(const at::Tensor& x, const at::Tensor& A, const at::Tensor& zi,
+                               const at::Tensor& table, at::Tensor& y, int64_t K, int64_t G,
+                               int64_t maxP, int64_t N, int64_t chunk_len, at::IntArrayRef extents) {
+  return ragged_forward(x, A, zi, table, y, K, G, maxP, N, chunk_len, extents, "lpc_ragged_keep_");
 }
 
 void lpc_ragged_backward_hip(const at::Tensor& g, const at::Tensor& A, const at::Tensor& zi,
                              const at::Tensor& y, const at::Tensor& table, at::Tensor& dx,
                              at::Tensor& dA, at::Tensor& dzi, int64_t K, int64_t G, int64_t maxP,
                              int64_t N, int64_t chunk_len, bool need_x, bool need_a,
-                             bool need_zi, at::IntArrayRef extents) {
+                             bool need_zi, at::IntArrayRef extents,
+                             const c10::optional<at::Tensor>& kept) {
   check_ragged(g, A, zi, table, K, G, extents, "lpc_ragged_backward_");
   TORCH_CHECK(y.sizes() == g.sizes() && y.is_contiguous(),
               "torchlpc_amd::lpc_ragged_backward_: y must be a contiguous tensor like g");
@@ -383,10 +399,22 @@ void lpc_ragged_backward_hip(const at::Tensor& g, const at::Tensor& A, const at:
   if (!(need_x || need_a || need_zi)) return;
   c10::DeviceGuard guard(g.device());
   const int dt = dtype_code(g);
-  const int flags = (need_x ? TLPC_NEED_DX : 0) | (need_a ? TLPC_NEED_DA : 0) |
-                    (need_zi ? TLPC_NEED_DZI : 0);
+  int flags = (need_x ? TLPC_NEED_DX : 0) | (need_a ? TLPC_NEED_DA : 0) |
+              (need_zi ? TLPC_NEED_DZI : 0);
   const size_t wsb = tlpc_lpc_ragged_workspace(dt, G, (int32_t)N);
-  auto ws = workspace(g, wsb);
+  at::Tensor ws;
+  if (kept.has_value() && kept->defined()) {
+    // the workspace of lpc_ragged_keep_ for this table: its summaries are reused
+    TORCH_CHECK(kept->scalar_type() == at::kByte && kept->is_contiguous() && kept->device() == g.device() &&
+                    (size_t)kept->numel() >= wsb,
+                "torchlpc_amd::lpc_ragged_backward_: `kept` must be the workspace lpc_ragged_keep_ returned "
+                "for this plan group (",
+                wsb, " bytes)");
+    ws = *kept;
+    flags |= TLPC_REUSE_SUMMARY;
+  } else {
+    ws = workspace(g, wsb);
+  }
   check_rc(tlpc_lpc_ragged_backward(dt, g.data_ptr(), A.data_ptr(), zi.data_ptr(), y.data_ptr(),
                                     need_x ? dx.data_ptr() : nullptr,
                                     need_a ? dA.data_ptr() : nullptr,
@@ -429,10 +457,14 @@ void lpc_ragged_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, con
                     at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t, at::IntArrayRef) {
   cpu_unsupported("lpc_ragged_");
 }
+at::Tensor lpc_ragged_keep_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, const at::Tensor&,
+                               at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t, at::IntArrayRef) {
+  cpu_unsupported("lpc_ragged_keep_");
+}
 void lpc_ragged_backward_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&,
                              const at::Tensor&, const at::Tensor&, at::Tensor&, at::Tensor&,
                              at::Tensor&, int64_t, int64_t, int64_t, int64_t, int64_t, bool, bool,
-                             bool, at::IntArrayRef) {
+                             bool, at::IntArrayRef, const c10::optional<at::Tensor>&) {
   cpu_unsupported("lpc_ragged_backward_");
 }
 at::Tensor scan_cpu(const at::Tensor&, const at::Tensor&, const at::Tensor&, int64_t) {
@@ -471,9 +503,12 @@ TORCH_LIBRARY(torchlpc_amd, m) {
       "lpc_ragged_(Tensor x, Tensor A, Tensor zi, Tensor table, Tensor(a!) y, int K, int G, "
       "int max_chunks, int N, int chunk_len, int[] extents) -> ()");
   m.def(
+      "lpc_ragged_keep_(Tensor x, Tensor A, Tensor zi, Tensor table, Tensor(a!) y, int K, int G, "
+      "int max_chunks, int N, int chunk_len, int[] extents) -> Tensor");
+  m.def(
       "lpc_ragged_backward_(Tensor g, Tensor A, Tensor zi, Tensor y, Tensor table, Tensor(a!) dx, "
       "Tensor(b!) dA, Tensor(c!) dzi, int K, int G, int max_chunks, int N, int chunk_len, "
-      "bool need_x, bool need_a, bool need_zi, int[] extents) -> ()");
+      "bool need_x, bool need_a, bool need_zi, int[] extents, Tensor(d!)? kept=None) -> ()");
   m.def("scan(Tensor x, Tensor w, Tensor init, int chunk_len=0) -> Tensor");
   m.def(
       "scan_backward(Tensor grad, Tensor w, Tensor init, Tensor h, bool need_x, bool need_w, "
@@ -487,6 +522,7 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CUDA, m) {
   m.impl("lpc_carries", &lpc_carries_hip);
   m.impl("lpc_backward_recompute", &lpc_backward_recompute_hip);
   m.impl("lpc_ragged_", &lpc_ragged_hip);
+  m.impl("lpc_ragged_keep_", &lpc_ragged_keep_hip);
   m.impl("lpc_ragged_backward_", &lpc_ragged_backward_hip);
   m.impl("scan", &scan_hip);
   m.impl("scan_backward", &scan_backward_hip);
@@ -500,6 +536,7 @@ TORCH_LIBRARY_IMPL(torchlpc_amd, CPU, m) {
   m.impl("lpc_backward_recompute", &lpc_backward_recompute_cpu);
   m.impl("lpc_ragged_plan", &lpc_ragged_plan_cpu);
   m.impl("lpc_ragged_", &lpc_ragged_cpu);
+  m.impl("lpc_ragged_keep_", &lpc_ragged_keep_cpu);
   m.impl("lpc_ragged_backward_", &lpc_ragged_backward_cpu);
   m.impl("scan", &scan_cpu);
   m.impl("scan_backward", &scan_backward_cpu);

@@ -7,7 +7,10 @@ The reference filters rectangular ``[B, T]`` batches with one order
 * ``a``:  ``(sum T_k * N_k,)``, signal k's coefficients as ``[T_k, N_k]`` rows,
 * ``zi``: ``(sum N_k,)``, signal k's initial state as in ``sample_wise_lpc``.
 
-Each output equals ``sample_wise_lpc`` on the signal alone. A :class:`RaggedPlan`
+Each output equals ``sample_wise_lpc`` on the signal alone. With gradients on,
+the float32 order-16 and order-32 groups keep their forward's workspace (saved
+for backward) and their backward reads its chunk transition matrices instead of
+forming them again. A :class:`RaggedPlan`
 holds the metadata (lengths, orders, offsets, and per order a device table that
 maps chunks to signals), built once on the host (``RaggedPlan.tune`` picks each
 group's chunk length from timing); the filter then runs as one set of
@@ -21,6 +24,7 @@ graph). ``RaggedPlan(..., streams=k)`` puts them on k side streams instead.
 """
 from __future__ import annotations
 
+import threading
 from typing import Any, List, Optional, Sequence, Tuple
 
 import torch
@@ -132,11 +136,16 @@ class RaggedPlan:
                 cand = grp if L == L0 else self._group_for(n, L)
 
                 def step(c=cand):
+                    # as the autograd Function runs it (kept summaries where they are reused)
                     n_, table, K, G, maxp, Lc, ext = c
-                    ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n_, Lc, ext)
+                    ws = None
+                    if grad is not None and keeps_summaries(c, x.dtype):
+                        ws = ops().lpc_ragged_keep_(x, a, zi, table, y, K, G, maxp, n_, Lc, ext)
+                    else:
+                        ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n_, Lc, ext)
                     if grad is not None:
                         ops().lpc_ragged_backward_(
-                            g, a, zi, y, table, dx, da, dzi, K, G, maxp, n_, Lc, True, True, True, ext
+                            g, a, zi, y, table, dx, da, dzi, K, G, maxp, n_, Lc, True, True, True, ext, ws
                         )
 
                 step()
@@ -160,9 +169,13 @@ class RaggedPlan:
     def run(self, fn) -> None:
         """Calls ``fn(group)`` for every order group: in order on the current
         stream, or on the plan's side streams when it was made with several."""
+        self.run_indexed(lambda gi, grp: fn(grp))
+
+    def run_indexed(self, fn) -> None:
+        """``run`` with ``fn(index, group)``."""
         if self.device.type != "cuda" or len(self.groups) < 2 or max(self.lanes) == 0:
-            for grp in self.groups:
-                fn(grp)
+            for gi, grp in enumerate(self.groups):
+                fn(gi, grp)
             return
         if self._streams is None:
             self._streams = [torch.cuda.Stream(device=self.device) for _ in range(max(self.lanes) + 1)]
@@ -171,7 +184,7 @@ class RaggedPlan:
             st.wait_stream(cur)
         for gi, grp in enumerate(self.groups):
             with torch.cuda.stream(self._streams[self.lanes[gi]]):
-                fn(grp)
+                fn(gi, grp)
         for st in self._streams:
             cur.wait_stream(st)
 
@@ -187,8 +200,26 @@ class RaggedPlan:
         return out
 
 
+# forward -> setup_context hand-over of the kept workspaces (same thread, back
+# to back: Function.apply runs setup_context right after forward)
+_kept = threading.local()
+
+
+def keeps_summaries(grp, dtype: torch.dtype) -> bool:
+    """Whether an order group's forward keeps its workspace for the backward:
+    float32 orders 16 and 32, whose backward then reads the forward's chunk
+    transition matrices instead of forming its own (the library ignores a kept
+    workspace elsewhere)."""
+    return dtype == torch.float32 and grp[0] in (16, 32)
+
+
 class RaggedLPC(Function):
     """Packed ``y`` of the ragged batch described by a :class:`RaggedPlan`."""
+
+    @classmethod
+    def apply(cls, *args, **kwargs):
+        _kept.grad_mode = torch.is_grad_enabled()
+        return Function.apply.__func__(cls, *args, **kwargs)
 
     @staticmethod
     def forward(x: torch.Tensor, a: torch.Tensor, zi: torch.Tensor, plan: RaggedPlan) -> torch.Tensor:
@@ -199,24 +230,46 @@ class RaggedLPC(Function):
                     f"made for, got {t.dtype} on {t.device}"
                 )
         y = torch.empty_like(x)
+        # Function.forward runs with grad mode off; apply recorded the caller's
+        caller_grad = getattr(_kept, "grad_mode", False)
+        _kept.grad_mode = False
+        keep = caller_grad and not torch.compiler.is_compiling() and (
+            x.requires_grad or a.requires_grad or zi.requires_grad
+        )
+        kept: List[Optional[torch.Tensor]] = [None] * len(plan.groups)
 
-        def one(grp):
+        def one(gi, grp):
             n, table, K, G, maxp, L, ext = grp
-            ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L, ext)
+            if keep and keeps_summaries(grp, x.dtype):
+                kept[gi] = ops().lpc_ragged_keep_(x, a, zi, table, y, K, G, maxp, n, L, ext)
+            else:
+                ops().lpc_ragged_(x, a, zi, table, y, K, G, maxp, n, L, ext)
 
-        plan.run(one)
+        plan.run_indexed(one)
+        _kept.item = (y, kept) if keep else None
         return y
 
     @staticmethod
     def setup_context(ctx: Any, inputs: Tuple[Any, ...], output: Any) -> None:
         _, a, zi, plan = inputs
-        ctx.save_for_backward(a, zi, output)
+        item = getattr(_kept, "item", None)
+        _kept.item = None
         ctx.plan = plan
+        ctx.kept_index = []
+        ws = []
+        if item is not None and item[0].data_ptr() == output.data_ptr():
+            # saved tensors like the others, so saved-tensor hooks see them
+            for gi, w in enumerate(item[1]):
+                if w is not None:
+                    ctx.kept_index.append(gi)
+                    ws.append(w)
+        ctx.save_for_backward(a, zi, output, *ws)
 
     @staticmethod
     @once_differentiable
     def backward(ctx: Any, grad_y: torch.Tensor):
-        a, zi, y = ctx.saved_tensors
+        a, zi, y = ctx.saved_tensors[:3]
+        kept = dict(zip(ctx.kept_index, ctx.saved_tensors[3:]))
         need_x, need_a, need_zi = ctx.needs_input_grad[:3]
         g = grad_y.contiguous()
         dx = torch.empty_like(g) if need_x else g.new_empty(0)
@@ -224,13 +277,13 @@ class RaggedLPC(Function):
         dzi = torch.empty_like(zi) if need_zi else zi.new_empty(0)
         if need_x or need_a or need_zi:
 
-            def one(grp):
+            def one(gi, grp):
                 n, table, K, G, maxp, L, ext = grp
                 ops().lpc_ragged_backward_(
-                    g, a, zi, y, table, dx, da, dzi, K, G, maxp, n, L, need_x, need_a, need_zi, ext
+                    g, a, zi, y, table, dx, da, dzi, K, G, maxp, n, L, need_x, need_a, need_zi, ext, kept.get(gi)
                 )
 
-            ctx.plan.run(one)
+            ctx.plan.run_indexed(one)
         return (dx if need_x else None, da if need_a else None, dzi if need_zi else None, None)
 
 
