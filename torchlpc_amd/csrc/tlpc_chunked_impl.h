@@ -2055,8 +2055,8 @@ bool ti_vec(const ChunkArgs<S>& a, const void* in, const void* out) {
 // streams (x or g, y, A, and the outputs that are present).
 template <typename S, int NP>
 constexpr bool kTv4 = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8);
-// The prefetching reruns of tlpc_grp_rerun.hip (NP / 4 lanes per chunk from
-// order 8 on, one below) exist for float32 orders 2, 4, 8, 16 and 32.
+// The prefetching multi-lane reruns (tlpc_grp_rerun.hip for orders 8, 16 and 32,
+// tlpc_quad_rerun.hip for orders 2 and 4) exist for float32.
 template <typename S, int NP>
 constexpr bool kGrp = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8 || NP == 16 || NP == 32);
 // The summaries on DPP row broadcasts (tlpc_row_summary.hip) exist for float32

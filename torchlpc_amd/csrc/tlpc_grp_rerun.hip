@@ -1,10 +1,10 @@
 // Reruns (phase 3 of the chunked scan) with prefetched streams and several
-// lanes per chunk, for float32 time-varying filters of order 2, 4, 8, 16 and 32
-// in uniform or ragged batches. Forward: G = NP / 4 lanes (one at orders 2 and
-// 4) share one chunk's serial recurrence, lane gl owning lags 4gl+1 .. 4gl+4,
-// so the G lanes of a chunk load a coefficient row as G float4 (one 16 * G-byte
-// piece; an order-2 row is one float2) and a wave's row loads cover 64 / G rows
-// instead of 64. Adjoint: NP / 2 lanes with two lags each. This is the quad
+// lanes per chunk, for float32 time-varying filters of order 8, 16 and 32 in
+// uniform or ragged batches (orders 2 and 4: tlpc_quad_rerun.hip). Forward:
+// G = NP / 4 lanes share one chunk's serial recurrence, lane gl owning lags
+// 4gl+1 .. 4gl+4, so the G lanes of a chunk load a coefficient row as G float4
+// (one 16 * G-byte piece) and a wave's row loads cover 64 / G rows instead of
+// 64. Adjoint: NP / 2 lanes with two lags each. This is the quad
 // scheme of the order-16 kernels (tlpc_dpp16.h) with the group width a template
 // parameter: partial sums meet through DPP (quad_perm within quads,
 // row_half_mirror across the two quads of an 8-lane group), lambda leaves the
@@ -29,15 +29,12 @@ namespace grp {
 using dpp16::qperm;
 using namespace vmem;
 
-// Forward group shape: G lanes per chunk, RW floats of a row per lane (an
-// order-2 row is padded to four coefficients in registers, the other two zero),
-// RING steps in flight per lane (a multiple of 4; deeper for the one-lane
-// groups, whose steps are short).
+// Forward group shape: G lanes per chunk, RING steps in flight per lane (a
+// multiple of 4).
 template <int NP>
 struct Grp {
-  static constexpr int G = NP >= 4 ? NP / 4 : 1;
-  static constexpr int RW = NP >= 4 ? 4 : NP;
-  static constexpr int RING = NP <= 4 ? 16 : 8;
+  static constexpr int G = NP / 4;
+  static constexpr int RING = 8;
 };
 constexpr int DPP_SHL1 = 0x101;     // row_shl:1  lane l reads lane l+1 of its 16-lane row
 constexpr int DPP_SHR1 = 0x111;     // row_shr:1  lane l reads lane l-1
@@ -47,7 +44,6 @@ constexpr int QP_PAIR0 = 0xA0;      // quad_perm [0,0,2,2]
 // sum over the G lanes of a group, in every lane of it
 template <int G>
 __device__ __forceinline__ double gsum(double p) {
-  if constexpr (G == 1) return p;
   p += qperm<dpp16::QP_SWAP1>(p);
   if constexpr (G >= 4) p += qperm<dpp16::QP_SWAP2>(p);
   // after the quad sums, lane 7-l of an 8-lane half holds the other quad's sum
@@ -58,9 +54,7 @@ __device__ __forceinline__ double gsum(double p) {
 // lane 0's value of a group, in every lane of it
 template <int G>
 __device__ __forceinline__ double gbcast0(double v, int gl) {
-  if constexpr (G == 1) {
-    return v;
-  } else if constexpr (G == 2) {
+  if constexpr (G == 2) {
     return qperm<QP_PAIR0>(v);
   } else if constexpr (G == 4) {
     return qperm<dpp16::QP_LANE0>(v);
@@ -71,23 +65,7 @@ __device__ __forceinline__ double gbcast0(double v, int gl) {
   }
 }
 
-// Lane's piece of a coefficient row (RW floats, zero-padded to four); AL: the
-// piece is aligned to its size.
-template <int NP, bool AL>
-__device__ __forceinline__ float4 ldrow(const float* p) {
-  if constexpr (Grp<NP>::RW == 4) {
-    return ldq<AL>(p);
-  } else if constexpr (AL) {
-    const float2 v = *reinterpret_cast<const float2*>(p);
-    return make_float4(v.x, v.y, 0.f, 0.f);
-  } else {
-    return make_float4(p[0], p[1], 0.f, 0.f);
-  }
-}
-template <int NP>
-__device__ __forceinline__ bool row_aligned(const void* p) {
-  return (reinterpret_cast<uintptr_t>(p) & (4 * Grp<NP>::RW - 1)) == 0;
-}
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 // y_u of one signal for the adjoint's window (y_{-1-k} = zi[k] for k < n, zero
 // past the order and at or past T): an always-valid address, and whether its
@@ -153,7 +131,7 @@ __device__ __forceinline__ void fwd_ring(double (&R)[4], float4 (&cq)[Grp<NP>::R
     const int64_t t = tb + JJ;
     const double y = fwd_step<JJ & 3, G>(R, cq[JJ], xq[JJ], gl);
     const int64_t tn = min(t + RING, T - 1);
-    cq[JJ] = ldrow<NP, AL>(Ab + tn * NP + 4 * gl);
+    cq[JJ] = ldq<AL>(Ab + tn * NP + 4 * gl);
     xq[JJ] = xb[tn];
     pin();
     // every lane of the group stores the same value; a step past the end stores
@@ -170,15 +148,14 @@ __device__ __forceinline__ void fwd_chunk(const float* Ab, const float* xb, floa
                                           int64_t T, int64_t t0, int len, int gl) {
   constexpr int RING = Grp<NP>::RING;
   double R[4];
-  // y_{t0-1-4gl-m} in slot 3-m (lags past an order-2 row meet zero coefficients)
 #pragma unroll
-  for (int m = 0; m < 4; ++m) R[3 - m] = m < NP ? cr[4 * gl + m] : 0.0;
+  for (int m = 0; m < 4; ++m) R[3 - m] = cr[4 * gl + m];  // y_{t0-1-4gl-m}, slot 3-m
   float4 cq[RING];
   float xq[RING];
 #pragma unroll
   for (int j = 0; j < RING; ++j) {
     const int64_t tc = min(t0 + j, T - 1);
-    cq[j] = opq(ldrow<NP, AL>(Ab + tc * NP + 4 * gl));
+    cq[j] = opq(ldq<AL>(Ab + tc * NP + 4 * gl));
     xq[j] = opq(xb[tc]);
   }
   settle(R);
@@ -200,15 +177,15 @@ __global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
   const int len = rem < a.L ? (int)rem : a.L;
   const float* Ab = a.A + sp.ao;
   const double* cr = a.carry + cid * NP;
-  if (row_aligned<NP>(Ab))
+  if (aligned16(Ab))
     fwd_chunk<NP, true>(Ab, a.x + sp.xo, a.y + sp.xo, cr, sp.T, t0, len, gl);
   else
     fwd_chunk<NP, false>(Ab, a.x + sp.xo, a.y + sp.xo, cr, sp.T, t0, len, gl);
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint, descending, with two lags per lane: G = NP / 2 lanes per chunk (one
-// at order 2), lane gl owning lags 2gl+1 and 2gl+2, so a group loads a row as G
+// Adjoint, descending, with two lags per lane: G = NP / 2 lanes per chunk,
+// lane gl owning lags 2gl+1 and 2gl+2, so a group loads a row as G
 // float2 and stores its dA row the same way. The adjoint needs no group sum,
 // only lambda broadcast from the group's lane 0, so twice the forward's lanes
 // per chunk cost one DPP move per step and give the latency-bound steps twice
@@ -218,7 +195,7 @@ __global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
 template <int NP>
 struct AdjGrp {
   static constexpr int G = NP / 2;
-  static constexpr int RING = NP <= 4 ? 16 : 8;  // steps in flight (the chunk length is a multiple)
+  static constexpr int RING = 8;  // steps in flight (the chunk length is a multiple)
 };
 
 template <int G>
@@ -466,14 +443,17 @@ bool grp_rerun_ok(const ChunkArgs<float>& a, int np) {
          a.rx == nullptr && (a.rg != nullptr || a.a_tstride == a.n) && a.L % 16 == 0;
 }
 
+// orders 2 and 4: one quad per chunk (tlpc_quad_rerun.hip)
+int launch_quad_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
+int launch_quad_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
+
 int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
-  const int G = np >= 4 ? np / 4 : 1;
+  if (np == 2 || np == 4) return launch_quad_fwd_rerun(a, np, st);
+  const int G = np / 4;
   const unsigned th = grp::grp_threads(a, G);
   const dim3 g = grp::grp_grid(a, G, th);
   if (g.x == 0) return 0;
   switch (np) {
-    case 2: hipLaunchKernelGGL(grp::grp_fwd_rerun<2>, g, dim3(th), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(grp::grp_fwd_rerun<4>, g, dim3(th), 0, st, a); break;
     case 8: hipLaunchKernelGGL(grp::grp_fwd_rerun<8>, g, dim3(th), 0, st, a); break;
     case 16: hipLaunchKernelGGL(grp::grp_fwd_rerun<16>, g, dim3(th), 0, st, a); break;
     case 32: hipLaunchKernelGGL(grp::grp_fwd_rerun<32>, g, dim3(th), 0, st, a); break;
@@ -500,20 +480,13 @@ int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st) {
 }
 
 int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
+  if (np == 2 || np == 4) return launch_quad_bwd_rerun(a, np, st);
   const int G = np / 2;
   const unsigned th = grp::grp_threads(a, G);
   const dim3 g = grp::grp_grid(a, G, th);
   if (g.x == 0) return 0;
   const bool da = a.dA != nullptr;
   switch (np) {
-    case 2:
-      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<2, true>), g, dim3(th), 0, st, a);
-      else hipLaunchKernelGGL((grp::grp_bwd_rerun<2, false>), g, dim3(th), 0, st, a);
-      break;
-    case 4:
-      if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<4, true>), g, dim3(th), 0, st, a);
-      else hipLaunchKernelGGL((grp::grp_bwd_rerun<4, false>), g, dim3(th), 0, st, a);
-      break;
     case 8:
       if (da) hipLaunchKernelGGL((grp::grp_bwd_rerun<8, true>), g, dim3(th), 0, st, a);
       else hipLaunchKernelGGL((grp::grp_bwd_rerun<8, false>), g, dim3(th), 0, st, a);
