@@ -54,19 +54,20 @@ inline size_t elem_size(int dtype) {
 }
 
 // Chunk length: a multiple of the staging block, at least 4*NP and at most 4096
-// samples, sized for about 32K rerun lanes: one lane per chunk, or NP/4 for
+// samples, sized for about 32K rerun lanes: one lane per chunk, NP/4 for
 // float32 orders 8, 16 and 32 that fill their padded width (n == NP; the group
-// reruns take whole coefficient rows), whose chunks are then NP/4 times longer
-// and their carry chains that much shorter. In the ragged workload's order
-// groups (about 3.3M samples each) the fastest lengths were 224 / 416 / 864
-// samples at orders 8 / 16 / 32 (0.32 / 0.39 / 0.81 ms per step), about 32K
-// lanes each, against 0.42 / 0.65 / 1.22 ms with 32K chunks. Summaries and carries are in the wide type (float64 for
-// single-precision data), so a chunk boundary adds no single-precision rounding
-// and shorter chunks cost no accuracy.
-inline bool multi_lane_rerun(int dtype, int n, int np) { return dtype == TLPC_F32 && np >= 8 && n == np; }
+// reruns take whole coefficient rows), and for float32 orders 2 and 4 (quads
+// per chunk) 16K chunks. Fewer, longer chunks shorten the carry chains. In the
+// ragged workload's order groups (about 3.3M samples each) the fastest lengths
+// were 256 / 224 / 224 / 416 / 864 samples at orders 2 / 4 / 8 / 16 / 32
+// (0.090 / 0.112 / 0.32 / 0.39 / 0.81 ms per step), against 0.097 / 0.131 /
+// 0.42 / 0.65 / 1.22 ms with 32K chunks. Summaries and carries are in the wide
+// type (float64 for single-precision data), so a chunk boundary adds no
+// single-precision rounding and shorter chunks cost no accuracy.
+inline bool multi_lane_rerun(int dtype, int n, int np) { return dtype == TLPC_F32 && n == np && np >= 2; }
 inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool multi_lane) {
   const int blk = chunk_block(np);
-  const int64_t target = multi_lane ? 32768 / (np / 4) : 32768;  // chunks
+  const int64_t target = !multi_lane ? 32768 : np >= 8 ? 32768 / (np / 4) : 16384;  // chunks
   int64_t L = requested > 0 ? requested : (B * T + target - 1) / target;
   if (requested <= 0 && L < 4 * np) L = 4 * np;
   if (L < blk) L = blk;

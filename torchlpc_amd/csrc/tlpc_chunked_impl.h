@@ -2059,10 +2059,10 @@ constexpr bool kTv4 = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP
 // tlpc_quad_rerun.hip for orders 2 and 4) exist for float32.
 template <typename S, int NP>
 constexpr bool kGrp = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8 || NP == 16 || NP == 32);
-// The summaries on DPP row broadcasts (tlpc_row_summary.hip) exist for float32
-// orders 16 and 32.
+// The multi-lane summaries exist for float32 orders 16 and 32 (DPP row
+// broadcasts, tlpc_row_summary.hip) and 2 and 4 (quads, tlpc_quad_rerun.hip).
 template <typename S, int NP>
-constexpr bool kRowSum = std::is_same<S, float>::value && (NP == 16 || NP == 32);
+constexpr bool kRowSum = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 16 || NP == 32);
 template <typename S>
 bool tv4_ok(const ChunkArgs<S>& a, int np, const void* y, const void* dx, const void* dA) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };

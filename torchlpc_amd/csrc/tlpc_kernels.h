@@ -152,8 +152,45 @@ int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
 // summaries (ChunkArgs::kept): float32 orders 16 and 32.
 bool grp_wsum_ok(const ChunkArgs<float>& a, int np);
 int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st);
-// Chunk summaries on DPP row broadcasts (tlpc_row_summary.hip): float32
-// time-varying orders 16 and 32 with n == NP, plain rows, L a multiple of NP.
+// The summary work item of a chunk's lanes in the multi-lane float32 summary
+// kernels, numbered as the LDS-staged summaries number them: item i of a
+// uniform batch is chunk (i / (P-1), i % (P-1)) (+1 for the adjoint); in a
+// ragged batch item i is chunk id i, and the chunks without a summary (a
+// signal's last forward, its first adjoint) are idle.
+struct SumItem {
+  const float* Ab;  // A at the chunk's first sample
+  const float* xb;  // x (forward) or g (adjoint) at the chunk's first sample
+  double* out;
+  int live;  // samples of the signal in the chunk
+};
+
+template <int NP, bool FWD>
+__device__ __forceinline__ bool summary_item(const ChunkArgs<float>& a, int64_t item, SumItem& it) {
+  int64_t k, c;
+  if (a.rg) {
+    if (item >= a.G) return false;
+    k = chunk_signal(a, item);
+    c = item - sig_span(a, k).cbase;
+  } else {
+    const int64_t cps = a.P - 1;
+    if (item >= a.B * cps) return false;
+    k = item / cps;
+    c = item - k * cps + (FWD ? 0 : 1);
+  }
+  const Span sp = sig_span(a, k);
+  if (FWD ? (c >= sp.P - 1) : (c < 1)) return false;
+  const int64_t t = c * a.L;
+  const int64_t rem = sp.T - t;
+  it.Ab = a.A + sp.ao + t * NP;
+  it.xb = a.x + sp.xo + t;
+  it.out = a.summ + item * (NP * NP + NP);
+  it.live = rem < a.L ? (int)rem : a.L;
+  return true;
+}
+
+// Multi-lane chunk summaries: float32 time-varying orders 16 and 32 on DPP
+// row broadcasts (tlpc_row_summary.hip), 2 and 4 on quads (tlpc_quad_rerun.hip);
+// n == NP, plain rows, L a multiple of max(NP, 16).
 bool row_summary_ok(const ChunkArgs<float>& a, int np);
 int launch_row_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st);
 

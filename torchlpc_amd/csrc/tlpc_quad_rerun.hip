@@ -1,6 +1,6 @@
-// Reruns (phase 3 of the chunked scan) for float32 time-varying filters of
-// order 2 and 4 in uniform or ragged batches, whatever the alignment of their
-// streams: one quad of lanes per chunk, lane q holding sample 4k + q of every
+// Reruns (phase 3 of the chunked scan) and chunk summaries (phase 1) for
+// float32 time-varying filters of order 2 and 4 in uniform or ragged batches,
+// whatever the alignment of their streams: one quad of lanes per chunk, lane q holding sample 4k + q of every
 // 4-step block k of the chunk (its x or g, its coefficient row, its entering y
 // value), and writing that sample's outputs. The quad's four loads of a block
 // touch consecutive addresses, one cache line (two across a line boundary),
@@ -303,7 +303,185 @@ inline dim3 quad_grid(const ChunkArgs<float>& a) {
   return dim3((unsigned)((lanes + THREADS - 1) / THREADS));
 }
 
+
+// ---------------------------------------------------------------------------
+// Chunk summaries at orders 2 and 4 (phase 1), one quad per chunk with the same
+// sample layout (lane q loads sample 4k + q of block k; each step broadcasts
+// its sample to the quad). Lane q owns basis index q mod NP (at order 2 lanes
+// 2 and 3 repeat lanes 0 and 1 and store nothing). Layouts as in the other
+// summaries: out[k * NP + r] = Phi[k][r] (Psi[k][r]), out[NP * NP + k] = z[k]
+// (w[k]).
+//   forward  descending, the transposed recurrence from a unit at output k
+//            (sample L-1-k): the pending sums left before the chunk are row k
+//            of Phi, and z[k] = sum_t lambda_t x_t. Forward summaries cover whole
+//            chunks.
+//   adjoint  ascending, the zero-input recurrence from a unit at the state
+//            entering the chunk (position -1-r): its final window is row r of
+//            Psi, and w[r] = sum_t g_t h_r(t); samples past the end of the
+//            signal have zero rows and g (selected at use).
+template <int NP>
+struct SumQ {
+  float vq[RB];      // x (forward) or g (adjoint) of the lane's sample
+  float aq[RB][NP];  // its row
+};
+
+template <int NP, int J>
+__device__ __forceinline__ void fsum_qstep(double (&p)[NP], double& z, float xv, const float (&row)[NP]) {
+  constexpr int S = (3 - J) & (NP - 1);  // slot of the step's position (block starts are 3 mod 4)
+  const double lam = p[S];
+  p[S] = 0.0;
+#pragma unroll
+  for (int i = 1; i <= NP; ++i) p[(S - i) & (NP - 1)] = fma(-(double)bq<J>(row[i - 1]), lam, p[(S - i) & (NP - 1)]);
+  z = fma(lam, (double)bq<J>(xv), z);
+}
+
+template <int NP, int SL>
+__device__ __forceinline__ void fsum_blocks(double (&p)[NP], double& z, SumQ<NP>& r, const float* Ab,
+                                            const float* xb, int tb, int q) {
+  if constexpr (SL < RB) {
+    const int t1 = tb - 4 * SL;  // samples t1 - j, j = 0..3; lane q's sample t1 - q
+    float row[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) row[i] = r.aq[SL][i];
+    const float xv = r.vq[SL];
+    fsum_qstep<NP, 0>(p, z, xv, row);
+    fsum_qstep<NP, 1>(p, z, xv, row);
+    fsum_qstep<NP, 2>(p, z, xv, row);
+    fsum_qstep<NP, 3>(p, z, xv, row);
+    {
+      const int tn = max(t1 - 4 * RB - q, 0);
+      r.vq[SL] = xb[tn];
+#pragma unroll
+      for (int i = 0; i < NP; ++i) r.aq[SL][i] = Ab[(int64_t)tn * NP + i];
+    }
+    pin();
+    fsum_blocks<NP, SL + 1>(p, z, r, Ab, xb, tb, q);
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(THREADS) void quad_fwd_summary(ChunkArgs<float> a) {
+  const int64_t gt = (int64_t)blockIdx.x * THREADS + threadIdx.x;
+  const int64_t item = gt >> 2;
+  const int q = (int)(gt & 3);
+  const int k = q & (NP - 1);
+  SumItem it;
+  if (!summary_item<NP, true>(a, item, it)) return;  // a quad's lanes leave together
+  const int L = a.L;
+  double p[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) p[i] = (i == NP - 1 - k) ? 1.0 : 0.0;  // unit at sample L-1-k
+  double z = 0.0;
+  SumQ<NP> r;
+#pragma unroll
+  for (int s = 0; s < RB; ++s) {
+    const int tn = max(L - 1 - 4 * s - q, 0);
+    r.vq[s] = opq(it.xb[tn]);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) r.aq[s][i] = opq(it.Ab[(int64_t)tn * NP + i]);
+  }
+  for (int jb = 0; jb < L; jb += 4 * RB) fsum_blocks<NP, 0>(p, z, r, it.Ab, it.xb, L - 1 - jb, q);
+  if (q < NP) {
+    // the pending sum of position -1-rr sits in slot NP-1-rr
+#pragma unroll
+    for (int rr = 0; rr < NP; ++rr) it.out[k * NP + rr] = p[NP - 1 - rr];
+    it.out[NP * NP + k] = z;
+  }
+}
+
+template <int NP, int J>
+__device__ __forceinline__ void asum_qstep(double (&h)[NP], double& w, float gv, const float (&row)[NP],
+                                           bool live) {
+  constexpr int S = J & (NP - 1);  // slot of the step's position (block starts are multiples of 4)
+  double acc = 0.0;
+#pragma unroll
+  for (int i = NP; i >= 1; --i) {
+    const float c = bq<J>(row[i - 1]);
+    acc = fma(-(double)(live ? c : 0.f), h[(S - i) & (NP - 1)], acc);
+  }
+  h[S] = acc;
+  const float g = bq<J>(gv);
+  w = fma((double)(live ? g : 0.f), acc, w);
+}
+
+template <int NP, int SL>
+__device__ __forceinline__ void asum_blocks(double (&h)[NP], double& w, SumQ<NP>& r, const float* Ab,
+                                            const float* gb, int live, int tb, int q) {
+  if constexpr (SL < RB) {
+    const int t1 = tb + 4 * SL;  // samples t1 + j; lane q's sample t1 + q
+    float row[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) row[i] = r.aq[SL][i];
+    const float gv = r.vq[SL];
+    asum_qstep<NP, 0>(h, w, gv, row, t1 < live);
+    asum_qstep<NP, 1>(h, w, gv, row, t1 + 1 < live);
+    asum_qstep<NP, 2>(h, w, gv, row, t1 + 2 < live);
+    asum_qstep<NP, 3>(h, w, gv, row, t1 + 3 < live);
+    {
+      const int tn = min(t1 + 4 * RB + q, live - 1);
+      r.vq[SL] = gb[tn];
+#pragma unroll
+      for (int i = 0; i < NP; ++i) r.aq[SL][i] = Ab[(int64_t)tn * NP + i];
+    }
+    pin();
+    asum_blocks<NP, SL + 1>(h, w, r, Ab, gb, live, tb, q);
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(THREADS) void quad_adj_summary(ChunkArgs<float> a) {
+  const int64_t gt = (int64_t)blockIdx.x * THREADS + threadIdx.x;
+  const int64_t item = gt >> 2;
+  const int q = (int)(gt & 3);
+  const int rr = q & (NP - 1);
+  SumItem it;
+  if (!summary_item<NP, false>(a, item, it)) return;  // a quad's lanes leave together
+  const int L = a.L, live = it.live;
+  double h[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) h[i] = (i == NP - 1 - rr) ? 1.0 : 0.0;  // unit at position -1-rr
+  double w = 0.0;
+  SumQ<NP> r;
+#pragma unroll
+  for (int s = 0; s < RB; ++s) {
+    const int tn = min(4 * s + q, live - 1);
+    r.vq[s] = opq(it.xb[tn]);  // the upstream gradient
+#pragma unroll
+    for (int i = 0; i < NP; ++i) r.aq[s][i] = opq(it.Ab[(int64_t)tn * NP + i]);
+  }
+  for (int jb = 0; jb < L; jb += 4 * RB) asum_blocks<NP, 0>(h, w, r, it.Ab, it.xb, live, jb, q);
+  if (q < NP) {
+    // Psi[rr][kk] = h_rr(L-1-kk), in slot NP-1-kk
+#pragma unroll
+    for (int kk = 0; kk < NP; ++kk) it.out[rr * NP + kk] = h[NP - 1 - kk];
+    it.out[NP * NP + rr] = w;
+  }
+}
+
+inline dim3 quad_sum_grid(const ChunkArgs<float>& a) {
+  const int64_t items = a.rg ? a.G : a.B * (a.P - 1);
+  return dim3((unsigned)((items * 4 + THREADS - 1) / THREADS));
+}
+
 }  // namespace quadr
+
+int launch_quad_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st) {
+  const dim3 g = quadr::quad_sum_grid(a);
+  if (g.x == 0) return 0;
+  const dim3 b(quadr::THREADS);
+  switch (np) {
+    case 2:
+      if (fwd) hipLaunchKernelGGL(quadr::quad_fwd_summary<2>, g, b, 0, st, a);
+      else hipLaunchKernelGGL(quadr::quad_adj_summary<2>, g, b, 0, st, a);
+      break;
+    case 4:
+      if (fwd) hipLaunchKernelGGL(quadr::quad_fwd_summary<4>, g, b, 0, st, a);
+      else hipLaunchKernelGGL(quadr::quad_adj_summary<4>, g, b, 0, st, a);
+      break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
 
 int launch_quad_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
   const dim3 g = quadr::quad_grid(a);

@@ -167,42 +167,6 @@ __device__ __forceinline__ double ff_step(double (&h)[NP], const double (&d)[NP 
 #undef TLPC_RS_ADJ16
 #undef TLPC_RS_FF16
 
-// The summary work item of a chunk's lanes, numbered as the LDS-staged
-// summaries number them: item i of a uniform batch is chunk (i / (P-1),
-// i % (P-1)) (+1 for the adjoint); in a ragged batch item i is chunk id i, and
-// the chunks without a summary (a signal's last forward, its first adjoint) are
-// idle.
-struct Item {
-  const float* Ab;  // A at the chunk's first sample
-  const float* xb;  // x (forward) or g (adjoint) at the chunk's first sample
-  double* out;
-  int live;  // samples of the signal in the chunk
-};
-
-template <int NP, bool FWD>
-__device__ __forceinline__ bool item_of(const ChunkArgs<float>& a, int64_t item, Item& it) {
-  int64_t k, c;
-  if (a.rg) {
-    if (item >= a.G) return false;
-    k = chunk_signal(a, item);
-    c = item - sig_span(a, k).cbase;
-  } else {
-    const int64_t cps = a.P - 1;
-    if (item >= a.B * cps) return false;
-    k = item / cps;
-    c = item - k * cps + (FWD ? 0 : 1);
-  }
-  const Span sp = sig_span(a, k);
-  if (FWD ? (c >= sp.P - 1) : (c < 1)) return false;
-  const int64_t t = c * a.L;
-  const int64_t rem = sp.T - t;
-  it.Ab = a.A + sp.ao + t * NP;
-  it.xb = a.x + sp.xo + t;
-  it.out = a.summ + item * (NP * NP + NP);
-  it.live = rem < a.L ? (int)rem : a.L;
-  return true;
-}
-
 template <int NP>
 __device__ __forceinline__ void to_f64(double (&d)[NP / 16][4], const float4 (&c)[NP / 16]) {
 #pragma unroll
@@ -250,7 +214,7 @@ __device__ __forceinline__ void fwd_block(double (&p)[NP], double& z, float4 (&c
 }
 
 template <int NP, bool AL>
-__device__ __forceinline__ void fwd_chunk(const Item& it, int L, int k) {
+__device__ __forceinline__ void fwd_chunk(const SumItem& it, int L, int k) {
   using Y = Lay<NP>;
   constexpr int NS = NP / 16;  // 16-step x pieces per block
   const int l = k & 15;
@@ -290,8 +254,8 @@ template <int NP>
 __global__ __launch_bounds__(THREADS) void row_fwd_summary(ChunkArgs<float> a) {
   const int64_t item = ((int64_t)blockIdx.x * THREADS + threadIdx.x) / NP;
   const int k = (int)(threadIdx.x & (NP - 1));
-  Item it;
-  if (!item_of<NP, true>(a, item, it)) return;  // a chunk's lanes leave together
+  SumItem it;
+  if (!summary_item<NP, true>(a, item, it)) return;  // a chunk's lanes leave together
   if ((reinterpret_cast<uintptr_t>(it.Ab) & 15) == 0)
     fwd_chunk<NP, true>(it, a.L, k);
   else
@@ -337,7 +301,7 @@ __device__ __forceinline__ void adj_block(double (&h)[NP], double& w, float4 (&c
 }
 
 template <int NP, bool AL>
-__device__ __forceinline__ void adj_chunk(const Item& it, int L, int r) {
+__device__ __forceinline__ void adj_chunk(const SumItem& it, int L, int r) {
   using Y = Lay<NP>;
   constexpr int NS = NP / 16;
   const int l = r & 15;
@@ -376,8 +340,8 @@ template <int NP>
 __global__ __launch_bounds__(THREADS) void row_adj_summary(ChunkArgs<float> a) {
   const int64_t item = ((int64_t)blockIdx.x * THREADS + threadIdx.x) / NP;
   const int r = (int)(threadIdx.x & (NP - 1));
-  Item it;
-  if (!item_of<NP, false>(a, item, it)) return;  // a chunk's lanes leave together
+  SumItem it;
+  if (!summary_item<NP, false>(a, item, it)) return;  // a chunk's lanes leave together
   if ((reinterpret_cast<uintptr_t>(it.Ab) & 15) == 0)
     adj_chunk<NP, true>(it, a.L, r);
   else
@@ -386,12 +350,16 @@ __global__ __launch_bounds__(THREADS) void row_adj_summary(ChunkArgs<float> a) {
 
 }  // namespace rsum
 
+// orders 2 and 4: one quad per chunk (tlpc_quad_rerun.hip)
+int launch_quad_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st);
+
 bool row_summary_ok(const ChunkArgs<float>& a, int np) {
-  return (np == 16 || np == 32) && a.n == np && !a.neg && (a.rg != nullptr || a.a_tstride == a.n) &&
-         a.L % np == 0 && a.L >= np;
+  return (np == 2 || np == 4 || np == 16 || np == 32) && a.n == np && !a.neg &&
+         (a.rg != nullptr || a.a_tstride == a.n) && a.L % (np < 16 ? 16 : np) == 0 && a.L >= np;
 }
 
 int launch_row_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st) {
+  if (np == 2 || np == 4) return launch_quad_summary(a, np, fwd, st);
   const int64_t items = a.rg ? a.G : a.B * (a.P - 1);
   const int64_t blocks = (items * np + rsum::THREADS - 1) / rsum::THREADS;
   if (blocks == 0) return 0;
