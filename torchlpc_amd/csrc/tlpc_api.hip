@@ -54,20 +54,21 @@ inline size_t elem_size(int dtype) {
 }
 
 // Chunk length: a multiple of the staging block, at least 4*NP and at most 4096
-// samples, sized for about 32K rerun lanes: one lane per chunk, NP/4 for
-// float32 orders 8, 16 and 32 that fill their padded width (n == NP; the group
-// reruns take whole coefficient rows), and for float32 orders 2 and 4 (quads
-// per chunk) 16K chunks. Fewer, longer chunks shorten the carry chains. In the
-// ragged workload's order groups (about 3.3M samples each) the fastest lengths
-// were 256 / 224 / 224 / 416 / 864 samples at orders 2 / 4 / 8 / 16 / 32
-// (0.090 / 0.112 / 0.32 / 0.39 / 0.81 ms per step), against 0.097 / 0.131 /
-// 0.42 / 0.65 / 1.22 ms with 32K chunks. Summaries and carries are in the wide
-// type (float64 for single-precision data), so a chunk boundary adds no
-// single-precision rounding and shorter chunks cost no accuracy.
+// samples, sized for a number of chunks: 32K with one rerun lane per chunk, and
+// for the float32 multi-lane kernels (orders 2 to 32 that fill their padded
+// width, n == NP) 16K at orders 2 and 4, 8K at orders 8 and 16 and 4K at order
+// 32, where longer chunks shorten the carry chains while the lanes per chunk
+// keep the reruns busy. These are the fastest lengths measured on the ragged
+// workload's order groups (about 3.3M samples each): 240 / 224 / 416 / 400 /
+// 832 samples at orders 2 / 4 / 8 / 16 / 32, 0.087 / 0.111 / 0.249 / 0.361 /
+// 0.639 ms per group step, against 0.095 / 0.131 / 0.37 / 0.61 / 1.06 ms at
+// 112-128 samples. Summaries and carries are in the wide type (float64 for
+// single-precision data), so a chunk boundary adds no single-precision rounding
+// and shorter chunks cost no accuracy.
 inline bool multi_lane_rerun(int dtype, int n, int np) { return dtype == TLPC_F32 && n == np && np >= 2; }
 inline int pick_chunk(int64_t B, int64_t T, int np, int requested, bool multi_lane) {
   const int blk = chunk_block(np);
-  const int64_t target = !multi_lane ? 32768 : np >= 8 ? 32768 / (np / 4) : 16384;  // chunks
+  const int64_t target = !multi_lane ? 32768 : np <= 4 ? 16384 : np <= 16 ? 8192 : 4096;  // chunks
   int64_t L = requested > 0 ? requested : (B * T + target - 1) / target;
   if (requested <= 0 && L < 4 * np) L = 4 * np;
   if (L < blk) L = blk;

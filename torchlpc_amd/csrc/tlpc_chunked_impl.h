@@ -445,13 +445,14 @@ __global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
     }
   };
   const bool own = wave == 0 && lane < NP;
-  // float64 chains of order 16 and 32 take the link's products through DPP row
-  // broadcasts of the state (no LDS round trip per link): order 16 on 16-lane
-  // rows (lane r owns s_r and row r), order 32 on 32-lane halves (lane e = 16h + l
-  // owns s_e and columns 16h..16h+15 of rows l and l + 16); the other lanes of
-  // the wave repeat them. Other orders and types broadcast the state through LDS.
-  constexpr bool kDpp = std::is_same<W, double>::value && (NP == 16 || NP == 32);
-  const int de = NP == 32 ? (lane & 31) : (lane & 15);  // DPP form: the state element of the lane
+  // float64 chains of order 8, 16 and 32 take the link's products through DPP
+  // row broadcasts of the state (no LDS round trip per link): orders 8 and 16
+  // on 16-lane rows (lane r owns s_r and row r; at order 8 lanes 8-15 repeat
+  // lanes 0-7), order 32 on 32-lane halves (lane e = 16h + l owns s_e and columns
+  // 16h..16h+15 of rows l and l + 16); the other lanes of the wave repeat them.
+  // Other orders and types broadcast the state through LDS.
+  constexpr bool kDpp = std::is_same<W, double>::value && (NP == 8 || NP == 16 || NP == 32);
+  const int de = NP == 32 ? (lane & 31) : (lane & (NP - 1));  // DPP form: the state element of the lane
   W s = Num<W>::zero();
   if constexpr (kDpp) {
     if (FWD && wave == 0 && de < a.n && a.zi) s = convert<W>(a.zi[sp.zo + de]);
@@ -492,10 +493,10 @@ __global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
           }
         }
       } else {
-        double m[2][16], zv[2];
-        auto ldm = [&](int li, double(&mm)[16], double& z) {
+        double m[2][NP], zv[2];
+        auto ldm = [&](int li, double(&mm)[NP], double& z) {
 #pragma unroll
-          for (int k = 0; k < 16; ++k) mm[k] = src[li][de * C::ROWP + k];
+          for (int k = 0; k < NP; ++k) mm[k] = src[li][de * C::ROWP + k];
           z = src[li][NP * C::ROWP + de];
         };
         ldm(0, m[0], zv[0]);
@@ -505,7 +506,10 @@ __global__ __launch_bounds__(256) void carry_pipe(ChunkArgs<S> a) {
           if (li < nb) {
             const int64_t l1 = l0 + li;
             if (own) carry[(FWD ? l1 : P - 1 - l1) * NP + lane] = s;
-            s = dpp16::chain_link(s, m[li & 1], zv[li & 1]);
+            if constexpr (NP == 16)
+              s = dpp16::chain_link(s, m[li & 1], zv[li & 1]);
+            else
+              s = dpp16::chain_link8(s, m[li & 1], zv[li & 1]);
           }
         }
       }
@@ -2060,9 +2064,9 @@ constexpr bool kTv4 = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP
 template <typename S, int NP>
 constexpr bool kGrp = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8 || NP == 16 || NP == 32);
 // The multi-lane summaries exist for float32 orders 16 and 32 (DPP row
-// broadcasts, tlpc_row_summary.hip) and 2 and 4 (quads, tlpc_quad_rerun.hip).
+// broadcasts, tlpc_row_summary.hip) and 2, 4 and 8 (quads, tlpc_quad_rerun.hip).
 template <typename S, int NP>
-constexpr bool kRowSum = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 16 || NP == 32);
+constexpr bool kRowSum = std::is_same<S, float>::value && (NP == 2 || NP == 4 || NP == 8 || NP == 16 || NP == 32);
 template <typename S>
 bool tv4_ok(const ChunkArgs<S>& a, int np, const void* y, const void* dx, const void* dA) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };

@@ -134,6 +134,26 @@ __device__ __forceinline__ double chain_link(double s, const double (&m)[NP], do
   return (acc0 + acc1) + (acc2 + acc3);
 }
 
+// Carry link of an 8-state chain on lanes 0-7 of each 16-lane row (lanes 8-15
+// repeat lanes 0-7): returns z_r + sum_k M[r][k] s_k, with s_k reaching every
+// lane through a row_newbcast operand (lane k); four accumulators of two
+// products each.
+__device__ __forceinline__ double chain_link8(double s, const double (&m)[8], double z) {
+  double acc0 = z, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %4, %5 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %4, %6 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %4, %7 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, %4, %8 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %4, %9 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %4, %10 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %4, %11 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, %4, %12 row_newbcast:7 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc0), "+v"(acc1), "+v"(acc2), "+v"(acc3)
+      : "v"(s), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]));
+  return (acc0 + acc1) + (acc2 + acc3);
+}
+
 // v of lane l ^ 16 (the other 16-lane row of the lane's 32-lane half), for a
 // lane whose row parity is h = (lane >> 4) & 1, without the LDS crossbar:
 // v_permlane16_swap exchanges the odd rows of its first operand with the even

@@ -189,7 +189,7 @@ __device__ __forceinline__ bool summary_item(const ChunkArgs<float>& a, int64_t 
 }
 
 // Multi-lane chunk summaries: float32 time-varying orders 16 and 32 on DPP
-// row broadcasts (tlpc_row_summary.hip), 2 and 4 on quads (tlpc_quad_rerun.hip);
+// row broadcasts (tlpc_row_summary.hip), 2, 4 and 8 on quads (tlpc_quad_rerun.hip);
 // n == NP, plain rows, L a multiple of max(NP, 16).
 bool row_summary_ok(const ChunkArgs<float>& a, int np);
 int launch_row_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st);

@@ -1,5 +1,6 @@
-// Reruns (phase 3 of the chunked scan) and chunk summaries (phase 1) for
-// float32 time-varying filters of order 2 and 4 in uniform or ragged batches,
+// Reruns (phase 3 of the chunked scan) for float32 time-varying filters of
+// order 2 and 4, and chunk summaries (phase 1) of order 2, 4 and 8, in uniform
+// or ragged batches,
 // whatever the alignment of their streams: one quad of lanes per chunk, lane q holding sample 4k + q of every
 // 4-step block k of the chunk (its x or g, its coefficient row, its entering y
 // value), and writing that sample's outputs. The quad's four loads of a block
@@ -305,12 +306,12 @@ inline dim3 quad_grid(const ChunkArgs<float>& a) {
 
 
 // ---------------------------------------------------------------------------
-// Chunk summaries at orders 2 and 4 (phase 1), one quad per chunk with the same
-// sample layout (lane q loads sample 4k + q of block k; each step broadcasts
-// its sample to the quad). Lane q owns basis index q mod NP (at order 2 lanes
-// 2 and 3 repeat lanes 0 and 1 and store nothing). Layouts as in the other
-// summaries: out[k * NP + r] = Phi[k][r] (Psi[k][r]), out[NP * NP + k] = z[k]
-// (w[k]).
+// Chunk summaries at orders 2, 4 and 8 (phase 1), one quad per chunk with the
+// same sample layout (lane q loads sample 4k + q of block k; each step
+// broadcasts its sample to the quad). Lane q owns the NB = max(NP / 4, 1) basis
+// indices q + 4m (at order 2 lanes 2 and 3 repeat lanes 0 and 1 and store
+// nothing). Layouts as in the other summaries: out[k * NP + r] = Phi[k][r]
+// (Psi[k][r]), out[NP * NP + k] = z[k] (w[k]).
 //   forward  descending, the transposed recurrence from a unit at output k
 //            (sample L-1-k): the pending sums left before the chunk are row k
 //            of Phi, and z[k] = sum_t lambda_t x_t. Forward summaries cover whole
@@ -321,33 +322,49 @@ inline dim3 quad_grid(const ChunkArgs<float>& a) {
 //            signal have zero rows and g (selected at use).
 template <int NP>
 struct SumQ {
+  static constexpr int NB = NP >= 4 ? NP / 4 : 1;  // basis vectors per lane
   float vq[RB];      // x (forward) or g (adjoint) of the lane's sample
   float aq[RB][NP];  // its row
 };
 
-template <int NP, int J>
-__device__ __forceinline__ void fsum_qstep(double (&p)[NP], double& z, float xv, const float (&row)[NP]) {
-  constexpr int S = (3 - J) & (NP - 1);  // slot of the step's position (block starts are 3 mod 4)
-  const double lam = p[S];
-  p[S] = 0.0;
+template <int NP>
+__device__ __forceinline__ int basis_of(int q, int m) {
+  return NP >= 4 ? q + 4 * m : (q & (NP - 1));
+}
+
+// Step J of block SL of a pass: a pass starts at L-1-16m (7 mod 8: chunk lengths
+// are multiples of 16), so its block SL starts at 7 - 4 SL mod 8.
+template <int NP, int SL, int J>
+__device__ __forceinline__ void fsum_qstep(double (&p)[SumQ<NP>::NB][NP], double (&z)[SumQ<NP>::NB], float xv,
+                                           const float (&row)[NP]) {
+  constexpr int S = (7 - 4 * SL - J) & (NP - 1);  // slot of the step's position
+  double c[NP];
 #pragma unroll
-  for (int i = 1; i <= NP; ++i) p[(S - i) & (NP - 1)] = fma(-(double)bq<J>(row[i - 1]), lam, p[(S - i) & (NP - 1)]);
-  z = fma(lam, (double)bq<J>(xv), z);
+  for (int i = 0; i < NP; ++i) c[i] = (double)bq<J>(row[i]);
+  const double xd = (double)bq<J>(xv);
+#pragma unroll
+  for (int m = 0; m < SumQ<NP>::NB; ++m) {
+    const double lam = p[m][S];
+    p[m][S] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= NP; ++i) p[m][(S - i) & (NP - 1)] = fma(-c[i - 1], lam, p[m][(S - i) & (NP - 1)]);
+    z[m] = fma(lam, xd, z[m]);
+  }
 }
 
 template <int NP, int SL>
-__device__ __forceinline__ void fsum_blocks(double (&p)[NP], double& z, SumQ<NP>& r, const float* Ab,
-                                            const float* xb, int tb, int q) {
+__device__ __forceinline__ void fsum_blocks(double (&p)[SumQ<NP>::NB][NP], double (&z)[SumQ<NP>::NB],
+                                            SumQ<NP>& r, const float* Ab, const float* xb, int tb, int q) {
   if constexpr (SL < RB) {
     const int t1 = tb - 4 * SL;  // samples t1 - j, j = 0..3; lane q's sample t1 - q
     float row[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) row[i] = r.aq[SL][i];
     const float xv = r.vq[SL];
-    fsum_qstep<NP, 0>(p, z, xv, row);
-    fsum_qstep<NP, 1>(p, z, xv, row);
-    fsum_qstep<NP, 2>(p, z, xv, row);
-    fsum_qstep<NP, 3>(p, z, xv, row);
+    fsum_qstep<NP, SL, 0>(p, z, xv, row);
+    fsum_qstep<NP, SL, 1>(p, z, xv, row);
+    fsum_qstep<NP, SL, 2>(p, z, xv, row);
+    fsum_qstep<NP, SL, 3>(p, z, xv, row);
     {
       const int tn = max(t1 - 4 * RB - q, 0);
       r.vq[SL] = xb[tn];
@@ -361,17 +378,21 @@ __device__ __forceinline__ void fsum_blocks(double (&p)[NP], double& z, SumQ<NP>
 
 template <int NP>
 __global__ __launch_bounds__(THREADS) void quad_fwd_summary(ChunkArgs<float> a) {
+  constexpr int NB = SumQ<NP>::NB;
   const int64_t gt = (int64_t)blockIdx.x * THREADS + threadIdx.x;
   const int64_t item = gt >> 2;
   const int q = (int)(gt & 3);
-  const int k = q & (NP - 1);
   SumItem it;
   if (!summary_item<NP, true>(a, item, it)) return;  // a quad's lanes leave together
   const int L = a.L;
-  double p[NP];
+  double p[NB][NP], z[NB];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) p[i] = (i == NP - 1 - k) ? 1.0 : 0.0;  // unit at sample L-1-k
-  double z = 0.0;
+  for (int m = 0; m < NB; ++m) {
+    const int k = basis_of<NP>(q, m);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) p[m][i] = (i == NP - 1 - k) ? 1.0 : 0.0;  // unit at sample L-1-k
+    z[m] = 0.0;
+  }
   SumQ<NP> r;
 #pragma unroll
   for (int s = 0; s < RB; ++s) {
@@ -382,41 +403,55 @@ __global__ __launch_bounds__(THREADS) void quad_fwd_summary(ChunkArgs<float> a) 
   }
   for (int jb = 0; jb < L; jb += 4 * RB) fsum_blocks<NP, 0>(p, z, r, it.Ab, it.xb, L - 1 - jb, q);
   if (q < NP) {
-    // the pending sum of position -1-rr sits in slot NP-1-rr
 #pragma unroll
-    for (int rr = 0; rr < NP; ++rr) it.out[k * NP + rr] = p[NP - 1 - rr];
-    it.out[NP * NP + k] = z;
+    for (int m = 0; m < NB; ++m) {
+      const int k = basis_of<NP>(q, m);
+      // the pending sum of position -1-rr sits in slot NP-1-rr
+#pragma unroll
+      for (int rr = 0; rr < NP; ++rr) it.out[k * NP + rr] = p[m][NP - 1 - rr];
+      it.out[NP * NP + k] = z[m];
+    }
   }
 }
 
-template <int NP, int J>
-__device__ __forceinline__ void asum_qstep(double (&h)[NP], double& w, float gv, const float (&row)[NP],
-                                           bool live) {
-  constexpr int S = J & (NP - 1);  // slot of the step's position (block starts are multiples of 4)
-  double acc = 0.0;
+// Step J of block SL of a pass: a pass starts at 16m, so its block SL starts at
+// 4 SL mod 8.
+template <int NP, int SL, int J>
+__device__ __forceinline__ void asum_qstep(double (&h)[SumQ<NP>::NB][NP], double (&w)[SumQ<NP>::NB], float gv,
+                                           const float (&row)[NP], bool live) {
+  constexpr int S = (4 * SL + J) & (NP - 1);  // slot of the step's position
+  double c[NP];
 #pragma unroll
-  for (int i = NP; i >= 1; --i) {
-    const float c = bq<J>(row[i - 1]);
-    acc = fma(-(double)(live ? c : 0.f), h[(S - i) & (NP - 1)], acc);
+  for (int i = 0; i < NP; ++i) {
+    const float v = bq<J>(row[i]);
+    c[i] = (double)(live ? v : 0.f);
   }
-  h[S] = acc;
-  const float g = bq<J>(gv);
-  w = fma((double)(live ? g : 0.f), acc, w);
+  const float gf = bq<J>(gv);
+  const double g = (double)(live ? gf : 0.f);
+#pragma unroll
+  for (int m = 0; m < SumQ<NP>::NB; ++m) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = NP; i >= 1; --i) acc = fma(-c[i - 1], h[m][(S - i) & (NP - 1)], acc);
+    h[m][S] = acc;
+    w[m] = fma(g, acc, w[m]);
+  }
 }
 
 template <int NP, int SL>
-__device__ __forceinline__ void asum_blocks(double (&h)[NP], double& w, SumQ<NP>& r, const float* Ab,
-                                            const float* gb, int live, int tb, int q) {
+__device__ __forceinline__ void asum_blocks(double (&h)[SumQ<NP>::NB][NP], double (&w)[SumQ<NP>::NB],
+                                            SumQ<NP>& r, const float* Ab, const float* gb, int live, int tb,
+                                            int q) {
   if constexpr (SL < RB) {
     const int t1 = tb + 4 * SL;  // samples t1 + j; lane q's sample t1 + q
     float row[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) row[i] = r.aq[SL][i];
     const float gv = r.vq[SL];
-    asum_qstep<NP, 0>(h, w, gv, row, t1 < live);
-    asum_qstep<NP, 1>(h, w, gv, row, t1 + 1 < live);
-    asum_qstep<NP, 2>(h, w, gv, row, t1 + 2 < live);
-    asum_qstep<NP, 3>(h, w, gv, row, t1 + 3 < live);
+    asum_qstep<NP, SL, 0>(h, w, gv, row, t1 < live);
+    asum_qstep<NP, SL, 1>(h, w, gv, row, t1 + 1 < live);
+    asum_qstep<NP, SL, 2>(h, w, gv, row, t1 + 2 < live);
+    asum_qstep<NP, SL, 3>(h, w, gv, row, t1 + 3 < live);
     {
       const int tn = min(t1 + 4 * RB + q, live - 1);
       r.vq[SL] = gb[tn];
@@ -430,17 +465,21 @@ __device__ __forceinline__ void asum_blocks(double (&h)[NP], double& w, SumQ<NP>
 
 template <int NP>
 __global__ __launch_bounds__(THREADS) void quad_adj_summary(ChunkArgs<float> a) {
+  constexpr int NB = SumQ<NP>::NB;
   const int64_t gt = (int64_t)blockIdx.x * THREADS + threadIdx.x;
   const int64_t item = gt >> 2;
   const int q = (int)(gt & 3);
-  const int rr = q & (NP - 1);
   SumItem it;
   if (!summary_item<NP, false>(a, item, it)) return;  // a quad's lanes leave together
   const int L = a.L, live = it.live;
-  double h[NP];
+  double h[NB][NP], w[NB];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) h[i] = (i == NP - 1 - rr) ? 1.0 : 0.0;  // unit at position -1-rr
-  double w = 0.0;
+  for (int m = 0; m < NB; ++m) {
+    const int rr = basis_of<NP>(q, m);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) h[m][i] = (i == NP - 1 - rr) ? 1.0 : 0.0;  // unit at position -1-rr
+    w[m] = 0.0;
+  }
   SumQ<NP> r;
 #pragma unroll
   for (int s = 0; s < RB; ++s) {
@@ -451,10 +490,14 @@ __global__ __launch_bounds__(THREADS) void quad_adj_summary(ChunkArgs<float> a) 
   }
   for (int jb = 0; jb < L; jb += 4 * RB) asum_blocks<NP, 0>(h, w, r, it.Ab, it.xb, live, jb, q);
   if (q < NP) {
-    // Psi[rr][kk] = h_rr(L-1-kk), in slot NP-1-kk
 #pragma unroll
-    for (int kk = 0; kk < NP; ++kk) it.out[rr * NP + kk] = h[NP - 1 - kk];
-    it.out[NP * NP + rr] = w;
+    for (int m = 0; m < NB; ++m) {
+      const int rr = basis_of<NP>(q, m);
+      // Psi[rr][kk] = h_rr(L-1-kk), in slot NP-1-kk
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) it.out[rr * NP + kk] = h[m][NP - 1 - kk];
+      it.out[NP * NP + rr] = w[m];
+    }
   }
 }
 
@@ -477,6 +520,10 @@ int launch_quad_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st) {
     case 4:
       if (fwd) hipLaunchKernelGGL(quadr::quad_fwd_summary<4>, g, b, 0, st, a);
       else hipLaunchKernelGGL(quadr::quad_adj_summary<4>, g, b, 0, st, a);
+      break;
+    case 8:
+      if (fwd) hipLaunchKernelGGL(quadr::quad_fwd_summary<8>, g, b, 0, st, a);
+      else hipLaunchKernelGGL(quadr::quad_adj_summary<8>, g, b, 0, st, a);
       break;
     default: return -1;
   }

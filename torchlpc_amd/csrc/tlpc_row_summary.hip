@@ -350,16 +350,16 @@ __global__ __launch_bounds__(THREADS) void row_adj_summary(ChunkArgs<float> a) {
 
 }  // namespace rsum
 
-// orders 2 and 4: one quad per chunk (tlpc_quad_rerun.hip)
+// orders 2, 4 and 8: one quad per chunk (tlpc_quad_rerun.hip)
 int launch_quad_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st);
 
 bool row_summary_ok(const ChunkArgs<float>& a, int np) {
-  return (np == 2 || np == 4 || np == 16 || np == 32) && a.n == np && !a.neg &&
+  return (np == 2 || np == 4 || np == 8 || np == 16 || np == 32) && a.n == np && !a.neg &&
          (a.rg != nullptr || a.a_tstride == a.n) && a.L % (np < 16 ? 16 : np) == 0 && a.L >= np;
 }
 
 int launch_row_summary(ChunkArgs<float> a, int np, bool fwd, hipStream_t st) {
-  if (np == 2 || np == 4) return launch_quad_summary(a, np, fwd, st);
+  if (np <= 8) return launch_quad_summary(a, np, fwd, st);
   const int64_t items = a.rg ? a.G : a.B * (a.P - 1);
   const int64_t blocks = (items * np + rsum::THREADS - 1) / rsum::THREADS;
   if (blocks == 0) return 0;
