@@ -341,12 +341,16 @@ def _offset_view(a):
 
 
 @pytest.mark.parametrize("N", [2, 4, 8])
-@pytest.mark.parametrize("T,chunk", [(4, 0), (8, 16), (64, 16), (100, 32), (1000, 0), (4096, 48), (2048, 0)])
+@pytest.mark.parametrize(
+    "T,chunk", [(4, 0), (8, 16), (64, 16), (100, 32), (1000, 0), (4096, 48), (2048, 0), (37, 16), (1001, 0)]
+)
 def test_float4_stream_reruns(N, T, chunk):
     """Float32 time-varying orders 2, 4 and 8 with T a multiple of 4 take the
     float4-stream reruns (chunk 0 reads zi through its y window); the same data
-    from views that start 4 bytes into their allocation take the per-sample
-    reruns. Both meet the FP64 oracle."""
+    from views that start 4 bytes into their allocation (x, g and A), and any T
+    that is not a multiple of 4, take the quad (orders 2 and 4) or group (order
+    8) kernels, whose last 4-step blocks run past the signal's end. All meet the
+    FP64 oracle."""
     rng = np.random.default_rng(N * 1000 + T + chunk)
     B = 3
     x, A, zi = stable_inputs(rng, B, T, N, np.float32)
@@ -355,9 +359,10 @@ def test_float4_stream_reruns(N, T, chunk):
     ref = O.lpc_backward(up(g, np.float32), up(A, np.float32), up(zi, np.float32), y64)
     for aligned in (True, False):
         view = dev if aligned else _offset_view
-        y = ops().lpc(view(x), dev(A), dev(zi), 1, chunk)
+        Av = view(A)
+        y = ops().lpc(view(x), Av, dev(zi), 1, chunk)
         assert rel(host(y), y64) <= 1e-5, aligned
-        outs = ops().lpc_backward(view(g), dev(A), dev(zi), y, True, True, True, 1, chunk)
+        outs = ops().lpc_backward(view(g), Av, dev(zi), y, True, True, True, 1, chunk)
         for name, o, r in zip(("dx", "dA", "dzi"), outs, ref):
             assert rel(host(o), r) <= 1e-5, (aligned, name, rel(host(o), r))
 

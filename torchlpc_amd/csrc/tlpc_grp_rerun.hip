@@ -67,42 +67,6 @@ __device__ __forceinline__ double gbcast0(double v, int gl) {
 
 __device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
-// y_u of one signal for the adjoint's window (y_{-1-k} = zi[k] for k < n, zero
-// past the order and at or past T): an always-valid address, and whether its
-// value counts.
-__device__ __forceinline__ const float* yw_addr(const float* yb, const float* zib, int n, int64_t T, int64_t u) {
-  const int64_t uc = u < T ? u : T - 1;
-  const int64_t k = -u - 1;
-  const float* zq = zib ? zib + (k < n ? k : n - 1) : yb;
-  return u >= 0 ? yb + uc : zq;
-}
-__device__ __forceinline__ bool yw_ok(const float* zib, int n, int64_t T, int64_t u) {
-  return u >= 0 ? u < T : (zib != nullptr && -u - 1 < n);
-}
-// The same two, without a branch, for the adjoint's step loop: a branch there
-// lets the wait-count pass drain the prefetch ring at its join, and so does a
-// select on a value right after its load, so the ring keeps the raw value and
-// the step that consumes it applies the select. zb is the signal's zi, or any
-// valid address when there is none (haszi false).
-// Both candidate offsets pass through an empty asm statement before the select
-// (on offsets, not pointers, which would lose their global address space), and
-// the flag is formed bitwise: otherwise the compiler turns the conditionals into
-// control flow.
-__device__ __forceinline__ const float* yw_ptr(const float* yb, const float* zb, int n, int64_t T, int64_t u) {
-  const int64_t k = -u - 1;
-  int64_t op = u < T ? u : T - 1;
-  int64_t on = k < n ? k : n - 1;
-  asm("" : "+v"(op), "+v"(on));
-  const bool pos = u >= 0;
-  return (pos ? yb : zb) + (pos ? op : on);
-}
-__device__ __forceinline__ bool yw_flag(bool haszi, int n, int64_t T, int64_t u) {
-  const int pos = u >= 0;
-  int ok = (pos & (int)(u < T)) | ((pos ^ 1) & (int)haszi & (int)(-u - 1 < n));
-  asm("" : "+v"(ok));
-  return ok != 0;
-}
-
 // ---------------------------------------------------------------------------
 // Forward. Lane gl's window R holds y_u for u in [t-4-4gl, t-1-4gl] at slot
 // u mod 4 (t0 is a multiple of 4, so step JJ of a ring pass has slot J = JJ & 3
@@ -303,8 +267,8 @@ __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
       const int64_t u = tstart - 1 - 2 * gl - m;
-      const float v = *yw_addr(io.yb, io.zib, io.n, io.T, u);
-      Y[m] = yw_ok(io.zib, io.n, io.T, u) ? v : 0.f;
+      const float v = *yw_ptr(io.yb, io.zb, io.n, io.T, u);
+      Y[m] = yw_flag(io.zib != nullptr, io.n, io.T, u) ? v : 0.f;
     }
   }
   constexpr int RING = AdjGrp<NP>::RING;

@@ -159,23 +159,8 @@ struct AdjIo {
   bool haszi;
 };
 
-// y_u of the adjoint's window: address (always valid) and whether it counts
-// (y_{-1-k} = zi[k] for k < n, zero past the order and at or past T); formed
-// without control flow, which would make the wait-count pass drain the ring.
-__device__ __forceinline__ const float* yaddr(const AdjIo& io, int64_t u) {
-  const int64_t k = -u - 1;
-  int64_t op = u < io.T ? u : io.T - 1;
-  int64_t on = k < io.n ? k : io.n - 1;
-  asm("" : "+v"(op), "+v"(on));
-  const bool pos = u >= 0;
-  return (pos ? io.yb : io.zb) + (pos ? op : on);
-}
-__device__ __forceinline__ bool yflag(const AdjIo& io, int64_t u) {
-  const int pos = u >= 0;
-  int ok = (pos & (int)(u < io.T)) | ((pos ^ 1) & (int)io.haszi & (int)(-u - 1 < io.n));
-  asm("" : "+v"(ok));
-  return ok != 0;
-}
+__device__ __forceinline__ const float* yaddr(const AdjIo& io, int64_t u) { return yw_ptr(io.yb, io.zb, io.n, io.T, u); }
+__device__ __forceinline__ bool yflag(const AdjIo& io, int64_t u) { return yw_flag(io.haszi, io.n, io.T, u); }
 
 template <int NP, bool DA, int J>
 __device__ __forceinline__ float adj_qstep(double (&P)[NP], float (&Y)[NP], float (&d)[NP], float gv,
