@@ -159,8 +159,58 @@ __global__ __launch_bounds__(256) void grp_fwd_rerun(ChunkArgs<float> a) {
 template <int NP>
 struct AdjGrp {
   static constexpr int G = NP / 2;
-  static constexpr int RING = 8;  // steps in flight (the chunk length is a multiple)
+  static constexpr int RING = 8;     // steps in flight, and whose outputs are written together
+  // orders 16 and 32 stage their outputs; at order 8 the stage's writes cost more
+  // issue slots than the light steps leave (order-8 adjoint of the ragged
+  // workload: 117 us with a lane's own stores, 168 us staged)
+  static constexpr bool STAGE = NP >= 16;
+  static constexpr int CPW = 64 / G;  // chunks per wave (one wave per workgroup)
+  // LDS staging of a ring turn's outputs (padded so that the chunks' rows fall
+  // on different banks): dA rows and dx values, and per chunk the region to write
+  static constexpr int SA = RING * NP + 2;
+  static constexpr int SX = RING + 1;
 };
+
+// A ring turn's outputs are staged in LDS and then written by the whole wave,
+// one chunk region after another, in 128-byte-aligned runs of 64 floats: each
+// write covers whole cache lines except at a region's two ends. Written a row
+// per chunk and step (32 to 128 bytes, at the packed rows' arbitrary alignment),
+// the partial-line write requests had bounded the adjoint: a write-only probe of
+// that pattern reached 2.0 TB/s with 32-byte rows, 3.2 TB/s with unaligned
+// 64-byte rows, against 5.5 TB/s sequential.
+template <int NP>
+struct AdjStage {
+  float dA[AdjGrp<NP>::CPW * AdjGrp<NP>::SA];
+  float dx[AdjGrp<NP>::CPW * AdjGrp<NP>::SX];
+  int64_t a0[AdjGrp<NP>::CPW];  // dA element of the region's first row
+  int64_t x0[AdjGrp<NP>::CPW];  // dx element of the region's first row
+  int nl[AdjGrp<NP>::CPW];      // rows of the region inside the signal
+};
+
+// The workgroup is one wave, whose LDS accesses complete in issue order: the
+// staging only needs the compiler to keep them in program order. (A workgroup
+// barrier would also wait for the ring's loads in flight.)
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Raw buffer over n floats at p: stores at byte offsets at or past n*4 are
+// dropped by the hardware's range check, which masks lanes without a branch
+// (a branch in the step loop's body lets the register allocator and the
+// wait-count pass drain the prefetch ring).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t float_buffer(float* p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, n * 4, 0x00020000);
+}
+constexpr int kDropped = (int)0x80000000;  // a byte offset past every buffer here
+__device__ __forceinline__ void buffer_store(float v, __amdgpu_buffer_rsrc_t r, int byte_off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, byte_off, 0, 0);
+}
+
 
 template <int G>
 __device__ __forceinline__ double gbcast0_adj(double v, int gl) {
@@ -194,20 +244,24 @@ struct AdjStreams {
   const float* yb;
   const float* zib;
   const float* zb;  // zib, or yb when there is no zi (an always-valid address)
-  float* dxb;
+  int64_t dxo;      // element of dx / dA (in a.dx / a.dA) at the signal's sample 0
+  int64_t dAo;
+  float* dxb;       // the signal's dx and dA, for the stores of an unstaged group
   float* dAb;
-  float* sink;  // the group's consumed carry slot: destination of the stores of steps past T
+  float* sink;      // the group's consumed carry slot: destination of the stores of steps past T
   int64_t T;
   int n;
+  int cw;           // the chunk's slot in its wave
 };
 
 // DX / DA: whether dx / dA are written (template arguments, so that the step
-// loop holds no branch).
+// loop holds no branch); sa / sx: the LDS staging of step tb's outputs (the
+// ring's later steps stage one row down each).
 template <int NP, bool AL, bool DA, bool DX, int JJ>
 __device__ __forceinline__ void adj_ring(double (&P)[2], double& now, float (&Y)[2],
                                          float2 (&cq)[AdjGrp<NP>::RING], float (&gq)[AdjGrp<NP>::RING],
                                          float (&yq)[AdjGrp<NP>::RING], const AdjStreams& io, int64_t tb,
-                                         int gl) {
+                                         int gl, float* sa, float* sx) {
   constexpr int RING = AdjGrp<NP>::RING;
   if constexpr (JJ < RING) {
     constexpr int G = AdjGrp<NP>::G;
@@ -236,22 +290,73 @@ __device__ __forceinline__ void adj_ring(double (&P)[2], double& now, float (&Y)
     }
     pin();
     const float lf = (float)lam;
-    if constexpr (DX) *(live ? io.dxb + t : io.sink) = lf;
+    constexpr bool STAGE = AdjGrp<NP>::STAGE;
+    if constexpr (DX) {
+      if constexpr (STAGE)
+        sx[-JJ] = lf;  // the group's lanes hold the same value
+      else
+        *(live ? io.dxb + t : io.sink) = lf;
+    }
     if constexpr (DA) {
       const float2 d = make_float2(-lf * Y[(J - 1) & 1], -lf * Y[J]);
-      st2<AL>(live ? io.dAb + t * NP + 2 * gl : io.sink + 2 * gl, d);
+      if constexpr (STAGE)
+        *reinterpret_cast<float2*>(sa - JJ * NP) = d;
+      else
+        st2<AL>(live ? io.dAb + t * NP + 2 * gl : io.sink + 2 * gl, d);
       // the window moves down one position: slot S takes y_{t-3-2gl} from the lane
       // above, the last lane y_{t-1-NP}
       const float ydown = qperm<DPP_SHL1>(Y[S]);
       Y[S] = gl == G - 1 ? ynew : ydown;
     }
-    adj_ring<NP, AL, DA, DX, JJ + 1>(P, now, Y, cq, gq, yq, io, tb, gl);
+    adj_ring<NP, AL, DA, DX, JJ + 1>(P, now, Y, cq, gq, yq, io, tb, gl, sa, sx);
   }
+}
+
+// Writes the ring turn's staged outputs (rows tb-RING+1 .. tb of every chunk of the
+// wave that lie in its signal; `live`: the chunk exists).
+template <int NP, bool DA, bool DX>
+__device__ __forceinline__ void adj_flush(const AdjStreams& io, int64_t tb, bool live, AdjStage<NP>* stg, float* dA,
+                                          float* dx) {
+  using Q = AdjGrp<NP>;
+  constexpr int RING = Q::RING;
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t r0 = tb - RING + 1;
+  const int64_t nl64 = io.T - r0;
+  const int nl = live ? (int)(nl64 < 0 ? 0 : nl64 > RING ? RING : nl64) : 0;
+  // the group's lanes write the same values
+  stg->a0[io.cw] = io.dAo + r0 * NP;
+  stg->x0[io.cw] = io.dxo + r0;
+  stg->nl[io.cw] = nl;
+  lds_order();
+#pragma unroll
+  for (int c = 0; c < Q::CPW; ++c) {
+    const int n = __builtin_amdgcn_readfirstlane(stg->nl[c]);
+    if constexpr (DA) {
+      // from the 128-byte line holding the region's first element
+      const int64_t g0 = uniform64(stg->a0[c]);
+      const int lo = (int)(g0 & 31);
+      const __amdgpu_buffer_rsrc_t r = float_buffer(dA + (g0 - lo), lo + n * NP);
+      constexpr int PIECES = (RING * NP + 31 + 63) / 64;
+#pragma unroll
+      for (int k = 0; k < PIECES; ++k) {
+        const int i = k * 64 + lane;
+        const int j = i - lo;
+        const float v = stg->dA[c * Q::SA + min(max(j, 0), RING * NP - 1)];
+        buffer_store(v, r, j >= 0 ? i * 4 : kDropped);
+      }
+    }
+    if constexpr (DX) {
+      const __amdgpu_buffer_rsrc_t r = float_buffer(dx + uniform64(stg->x0[c]), n);
+      const int k = lane & (RING - 1);
+      buffer_store(stg->dx[c * Q::SX + k], r, lane < RING ? k * 4 : kDropped);
+    }
+  }
+  lds_order();
 }
 
 template <int NP, bool AL, bool DA, bool DX>
 __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr, int64_t t0, int L, int gl,
-                                          float* dz) {
+                                          float* dz, bool live, AdjStage<NP>* stg, float* dA, float* dx) {
   // tstart = t0 + L - 1 is odd: position tstart - (2gl+1+m) sits in slot m & 1
   // (m = 0, 1), in P as in Y
   double P[2];
@@ -283,9 +388,19 @@ __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr
     if constexpr (DA) yq[j] = opq(*yw_ptr(io.yb, io.zb, io.n, io.T, tstart - j - 1 - NP));
   }
   settle(P);
-  for (int jb = 0; jb < L; jb += RING)
-    adj_ring<NP, AL, DA, DX, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl);
-  if (dz) {
+  if constexpr (AdjGrp<NP>::STAGE) {
+    // staging of the turn's last row (rows ascend with t)
+    float* const sa = &stg->dA[io.cw * AdjGrp<NP>::SA + (RING - 1) * NP + 2 * gl];
+    float* const sx = &stg->dx[io.cw * AdjGrp<NP>::SX + RING - 1];
+    for (int jb = 0; jb < L; jb += RING) {
+      adj_ring<NP, AL, DA, DX, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl, sa, sx);
+      adj_flush<NP, DA, DX>(io, tstart - jb, live, stg, dA, dx);
+    }
+  } else {
+    for (int jb = 0; jb < L; jb += RING)
+      adj_ring<NP, AL, DA, DX, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl, nullptr, nullptr);
+  }
+  if (dz && live) {
     // pending sums of positions -1-r: r = 0 in `now` (lane 0), 2gl+1+m in slot m
     if (gl == 0) dz[0] = (float)now;
 #pragma unroll
@@ -296,41 +411,66 @@ __device__ __forceinline__ void adj_chunk(const AdjStreams& io, const double* cr
   }
 }
 
+template <int NP>
+__device__ __forceinline__ AdjStage<NP>* adj_stage() {
+  __shared__ AdjStage<NP> stg;
+  return &stg;
+}
+
+// One wave per workgroup (the staging is per wave). With staging every lane
+// stays to the end, since the wave writes its chunks' outputs together: lanes
+// past the last chunk run the last chunk again and write nothing.
 template <int NP, bool DA>
-__global__ __launch_bounds__(256) void grp_bwd_rerun(ChunkArgs<float> a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void grp_bwd_rerun(ChunkArgs<float> a) {
   constexpr int G = AdjGrp<NP>::G;
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t cid = gt / G;
-  const int gl = (int)(gt - cid * G);
-  if (cid >= (a.rg ? a.G : a.B * a.P)) return;  // a group's lanes leave together
+  constexpr bool STAGE = AdjGrp<NP>::STAGE;
+  const int64_t gt = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const int64_t nchunks = a.rg ? a.G : a.B * a.P;
+  const int64_t cid0 = gt / G;
+  const bool live = cid0 < nchunks;
+  if (!STAGE && !live) return;  // a group's lanes leave together
+  const int64_t cid = live ? cid0 : nchunks - 1;
+  const int gl = (int)(gt - cid0 * G);
   const int64_t b = chunk_signal(a, cid);
   const Span sp = sig_span(a, b);
   const int64_t c = cid - sp.cbase;
   double* cr = a.carry + cid * NP;
+  AdjStage<NP>* stg = nullptr;
+  if constexpr (STAGE) stg = adj_stage<NP>();
   AdjStreams io;
   io.Ab = a.A + sp.ao;
   io.gb = a.x + sp.xo;  // upstream gradient
   io.yb = a.y + sp.xo;
   io.zib = a.zi ? a.zi + sp.zo : nullptr;
   io.zb = io.zib ? io.zib : io.yb;
+  io.dxo = sp.xo;
+  io.dAo = sp.dao;
   io.dxb = a.dx ? a.dx + sp.xo : nullptr;
   io.dAb = DA ? a.dA + sp.dao : nullptr;
   // the carry was read before any store: its NP doubles hold the group's 2G floats
   io.sink = reinterpret_cast<float*>(cr);
   io.T = sp.T;
   io.n = a.n;
+  io.cw = (int)(threadIdx.x / G);
   float* dz = (c == 0 && a.dzi) ? a.dzi + sp.zo : nullptr;
-  const bool al = ((reinterpret_cast<uintptr_t>(io.Ab) | (DA ? reinterpret_cast<uintptr_t>(io.dAb) : 0)) & 7) == 0;
-  if (io.dxb) {
+  // Staged: one variant for the whole wave, whose lanes must not diverge, since
+  // each flush writes every chunk of the wave from the region bounds its lanes
+  // stage (a ragged wave's chunks belong to signals whose rows start at
+  // different alignments, so the aligned loads are taken only when every chunk
+  // allows them). Unstaged: the lane's own loads and dA stores.
+  const bool al =
+      STAGE ? __builtin_amdgcn_ballot_w64((reinterpret_cast<uintptr_t>(io.Ab) & 7) != 0) == 0
+            : ((reinterpret_cast<uintptr_t>(io.Ab) | (DA ? reinterpret_cast<uintptr_t>(io.dAb) : 0)) & 7) == 0;
+  if (a.dx) {
     if (al)
-      adj_chunk<NP, true, DA, true>(io, cr, c * a.L, a.L, gl, dz);
+      adj_chunk<NP, true, DA, true>(io, cr, c * a.L, a.L, gl, dz, live, stg, a.dA, a.dx);
     else
-      adj_chunk<NP, false, DA, true>(io, cr, c * a.L, a.L, gl, dz);
+      adj_chunk<NP, false, DA, true>(io, cr, c * a.L, a.L, gl, dz, live, stg, a.dA, a.dx);
   } else {
     if (al)
-      adj_chunk<NP, true, DA, false>(io, cr, c * a.L, a.L, gl, dz);
+      adj_chunk<NP, true, DA, false>(io, cr, c * a.L, a.L, gl, dz, live, stg, a.dA, a.dx);
     else
-      adj_chunk<NP, false, DA, false>(io, cr, c * a.L, a.L, gl, dz);
+      adj_chunk<NP, false, DA, false>(io, cr, c * a.L, a.L, gl, dz, live, stg, a.dA, a.dx);
   }
 }
 
@@ -357,7 +497,7 @@ __device__ __forceinline__ void wsum_chunk(const AdjStreams& io, int64_t t0, int
     yq[j] = 0.f;
   }
   for (int jb = 0; jb < L; jb += RING)
-    adj_ring<NP, AL, false, false, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl);
+    adj_ring<NP, AL, false, false, 0>(P, now, Y, cq, gq, yq, io, tstart - jb, gl, nullptr, nullptr);
   // pending sums of positions t0-1-r: r = 0 in `now` (lane 0), 2gl+1+m in slot m
   if (gl == 0) out[0] = now;
 #pragma unroll
