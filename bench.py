@@ -459,6 +459,7 @@ def bench_workload(name, steps, warmup, ragged_order=0, tune=True):
 
 def _short(name):
     """Kernel name without namespace, return type and argument list."""
+    name = name.replace("(anonymous namespace)::", "")
     name = name.split("(")[0].replace("void ", "").strip()
     return name.split("::")[-1]
 

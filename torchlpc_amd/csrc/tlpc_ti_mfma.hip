@@ -452,20 +452,24 @@ __global__ __launch_bounds__(NT, 2) void ti_mfma(ChunkArgs<float> a) {
         out_tiles<1, 2>(out, hop, pop, xs, cs + ct * CT * DP, 16 * cb + l16, lq);
       else
         out_tiles<0, 3>(out, hop, pop, xs, cs + ct * CT * DP, 16 * cb + l16, lq);
+      // The outputs leave through the staging region in sample order: straight
+      // from the MFMA layout a store instruction would cover 16 blocks with four
+      // floats each, a cache request per 16 bytes.
+      if (!BWD) {
+        __syncthreads();  // every wave has read the X staging
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int rb = (wave & 1) ? (q == 0 ? 1 : 2) : (q == 0 ? 0 : 3);
-        const int j = 16 * cb + l16;
+        for (int q = 0; q < 2; ++q) {
+          const int rb = (wave & 1) ? (q == 0 ? 1 : 2) : (q == 0 ? 0 : 3);
+          const int j = 16 * cb + l16;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * rb + 4 * lq + r;
-          const int64_t t = t0 + (int64_t)j * LB + i;
-          if (t < T) {
-            if (!BWD)
-              a.y[base + t] = out[q][r];
-            else if (a.dx)
-              a.dx[base + T - 1 - t] = out[q][r];
-          }
+          for (int r = 0; r < 4; ++r) xs[j * XP + 16 * rb + 4 * lq + r] = out[q][r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < CTS / NT; ++e) {
+          const int sl = e * NT + tid;
+          const int64_t t = t0 + sl;
+          if (t < T) a.y[base + t] = xs[(sl >> 6) * XP + (sl & 63)];
         }
       }
       if (BWD) {
@@ -485,6 +489,15 @@ __global__ __launch_bounds__(NT, 2) void ti_mfma(ChunkArgs<float> a) {
           }
         }
         __syncthreads();
+        if (a.dx) {
+          // dx_t = lambda_t, from the staged lambda' in sample order (descending t)
+#pragma unroll
+          for (int e = 0; e < CTS / NT; ++e) {
+            const int tl = e * NT + tid;
+            const int64_t t = t0 + tl;
+            if (t < T) a.dx[base + T - 1 - t] = xs[(tl >> 5) * WP + (tl & 31)];
+          }
+        }
         if (with_da) {
           // M[r][m] += sum_q lambda'_{32q+r} y'_{32q+m}, q over the tile's 64 rows
 #pragma unroll
