@@ -77,11 +77,18 @@ constexpr int FC = FS / LC;
 constexpr int FSX = FS / 4;
 static_assert(FC % 4 == 0 && FC <= 16 && (FS * NP / 4) % STAGERS == 0 && FSX <= STAGERS, "forward tile layout");
 
+// The forward keeps four tiles of rows in LDS and two of summaries, so that no
+// buffer a wave writes in a period is read in it: tile p+1's rows go where tile
+// p-3's were (its rerun ended the period before), tile p's summaries where
+// tile p-2's were (the chain read them the period before). Each wave then
+// commits its staged loads, and the sweeping waves their summaries, as soon as
+// its own stage is done, and a period ends at a single barrier (156 KB of LDS).
+constexpr int FNBUF = 4;
 struct FwdLds {
-  float4 A[NBUF][FS * 4];  // tile rows: A[b][r*4 + q] = A[t0 + r][4q .. 4q+3]
-  float4 x[NBUF][FSX];     // x[t0 .. t0+FS)
-  double summ[FC][LINKP];  // Phi rows (padded) then z, of the tile in stage 2
-  double carry[2][FC][NP]; // state entering each sub-chunk (written in stage 2, read in 3)
+  float4 A[FNBUF][FS * 4];    // tile rows: A[b][r*4 + q] = A[t0 + r][4q .. 4q+3]
+  float4 x[FNBUF][FSX];       // x[t0 .. t0+FS)
+  double summ[2][FC][LINKP];  // Phi rows (padded) then z: tile p's in summ[p & 1]
+  double carry[2][FC][NP];    // state entering each sub-chunk (written in stage 2, read in 3)
 };
 
 struct BwdLds {
@@ -574,7 +581,7 @@ __global__ __launch_bounds__(THREADS) void tile_fwd(ChunkArgs<float> a) {
 #pragma unroll 1
   for (int64_t p = 0; p < ntiles + 2; ++p) {
     const bool more = p + 1 < ntiles;
-    const int nb = (int)((p + 1) % NBUF);
+    const int nb = (int)((p + 1) % FNBUF);
     // staged registers live within one period (not carried across the loop)
     // the loads are unconditional (in the last periods they read the last tile
     // again, never committed): a conditional load is waited for at the merge
@@ -584,17 +591,16 @@ __global__ __launch_bounds__(THREADS) void tile_fwd(ChunkArgs<float> a) {
     double ph[NP], z = 0.0;
     const int kk = lane & 15, c = wave * 4 + (lane >> 4);
     if (wave < FWD_PHI_WAVES && p < ntiles) {
-      const int bb = (int)(p % NBUF);
+      const int bb = (int)(p % FNBUF);
       phi_sweep<true>(L.A[bb], reinterpret_cast<const float*>(L.x[bb]), c, kk, ph, z);
+      phi_store<true>(L.summ[p & 1][c], kk, ph, z);
     } else if (wave == FWD_CHAIN_WAVE && p >= 1 && p <= ntiles) {
-      chain<true, FC>(s, L.summ, L.carry[(p - 1) & 1], lane);
+      chain<true, FC>(s, L.summ[(p - 1) & 1], L.carry[(p - 1) & 1], lane);
     } else if (wave == FWD_RERUN_WAVE && p >= 2) {
       const int64_t k = p - 2;
-      const int bb = (int)(k % NBUF);
+      const int bb = (int)(k % FNBUF);
       fwd_rerun<ALIGNED, FS>(L.A[bb], L.x[bb], L.carry[k & 1], yb, T, k * FS, lane);
     }
-    lds_barrier();
-    if (wave < FWD_PHI_WAVES && p < ntiles) phi_store<true>(L.summ[c], kk, ph, z);
     st.consume_all();
     if (more) {
       st.commit_rows(L.A[nb], T, si);
