@@ -26,12 +26,14 @@
 //   stage 4 (tile p-3)  backward, one wave: dx, the 64-byte dA rows and, at the
 //                       start of the signal, dzi.
 //
-// A tile's A rows and per-sample streams sit in LDS from stage 1 to stage 3
-// (three buffers); the next tile's global loads are issued into registers when a
-// period starts and written to LDS when it ends, so they overlap the period's
-// arithmetic. Stage 1 keeps its summaries in registers until the end of the
-// period, when stage 2 is done with the previous tile's, so one summary buffer
-// serves both.
+// A tile's A rows and per-sample streams sit in LDS from stage 1 to stage 3;
+// the next tile's global loads are issued into registers when a period starts
+// and written to LDS later, so they overlap the period's arithmetic. The
+// backward (three row buffers, one summary buffer) writes them, and stage 1 its
+// summaries, after a barrier at the end of the period, when stage 2 is done
+// with the previous tile's summaries. The forward (four row buffers, two summary
+// buffers) has each wave write them as soon as its own stage is done, and ends
+// a period at one barrier.
 //
 // Samples past the end of the signal are zero rows and zero inputs in LDS, so
 // every sub-chunk is whole: the forward computes values there that it does not
