@@ -258,3 +258,39 @@ def test_ragged_kept_summaries_backward():
         torch.cuda.synchronize()
         for e, o in zip(eager, outs):
             assert torch.equal(e, o), r
+
+
+def test_ragged_backward_partial_gradients():
+    """dx only, dA only, dzi only and dx with dzi: every requested output equals
+    the full backward's bit for bit, at orders 2 to 32 (the adjoint variants
+    without dx or dA stores, in the quad and group kernels and the staged
+    writes of orders 16 and 32), with chunks ending past a signal's end and rows
+    that are not 16-byte aligned. The full backward is checked against FP64."""
+    from torchlpc_amd import RaggedPlan
+
+    rng = np.random.default_rng(31)
+    lengths = [7, 700, 3001, 64, 1025, 5000, 333, 2048, 129]
+    orders = [1, 2, 16, 32, 8, 32, 4, 16, 8]
+    sigs = _mixed(rng, lengths, orders)
+    x, a, zi, g = _pack(sigs, np.float32)
+    X, AA, Z, Gd = dev(x), dev(a), dev(zi), dev(g)
+    plan = RaggedPlan(lengths, orders, X.device, X.dtype, chunk_len=128)
+    ops = torch.ops.torchlpc_amd
+    y = torch.empty_like(X)
+    for n, table, K, G, maxp, L, ext in plan.groups:
+        ops.lpc_ragged_(X, AA, Z, table, y, K, G, maxp, n, L, ext)
+
+    def backward(need_x, need_a, need_zi):
+        outs = [torch.full_like(t, float("nan")) for t in (X, AA, Z)]
+        for n, table, K, G, maxp, L, ext in plan.groups:
+            ops.lpc_ragged_backward_(Gd, AA, Z, y, table, *outs, K, G, maxp, n, L, need_x, need_a, need_zi, ext,
+                                     None)
+        return outs
+
+    full = backward(True, True, True)
+    _check(sigs, np.float32, plan, y, *full, REL_TOL[np.float32])
+    for flags in ((True, False, False), (False, True, False), (False, False, True), (True, False, True)):
+        part = backward(*flags)
+        for want, f, p in zip(flags, full, part):
+            if want:
+                assert torch.equal(f, p), flags
