@@ -181,7 +181,7 @@ size_t tlpc_lpc_summary_bytes(int dtype, int64_t B, int64_t T, int32_t N, int32_
  *
  * tlpc_lpc_ragged_backward takes TLPC_REUSE_SUMMARY in `flags` when its workspace
  * still holds what tlpc_lpc_ragged_forward wrote there for the same table, A,
- * N, dtype and chunk length: for float32 orders 16 and 32 the backward then
+ * N, dtype and chunk length: for float32 orders 8, 16 and 32 the backward then
  * reads the forward's chunk transition matrices (transposed) instead of forming
  * its own, and overwrites only the rest of the workspace, so the flag may be
  * given again to a later backward with the same workspace. It is ignored for

@@ -185,7 +185,7 @@ def test_ragged_tune_per_group_chunk_lengths():
 
 
 def test_ragged_kept_summaries_backward():
-    """With gradients on, the float32 order-16 and order-32 groups keep their
+    """With gradients on, the float32 order-8, order-16 and order-32 groups keep their
     forward's workspace and the backward reads the forward's chunk transition
     matrices (transposed) instead of forming its own: the gradients match the
     FP64 oracle and the backward that forms its own summaries, including chunks

@@ -567,7 +567,7 @@ int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st) {
 }
 
 bool grp_wsum_ok(const ChunkArgs<float>& a, int np) {
-  return (np == 16 || np == 32) && a.rg != nullptr && grp_rerun_ok(a, np);
+  return (np == 8 || np == 16 || np == 32) && a.rg != nullptr && grp_rerun_ok(a, np);
 }
 
 int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st) {
@@ -576,6 +576,7 @@ int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st) {
   const dim3 g = grp::grp_grid(a, G, th);
   if (g.x == 0) return 0;
   switch (np) {
+    case 8: hipLaunchKernelGGL(grp::grp_wsum<8>, g, dim3(th), 0, st, a); break;
     case 16: hipLaunchKernelGGL(grp::grp_wsum<16>, g, dim3(th), 0, st, a); break;
     case 32: hipLaunchKernelGGL(grp::grp_wsum<32>, g, dim3(th), 0, st, a); break;
     default: return -1;

@@ -149,7 +149,7 @@ bool grp_rerun_ok(const ChunkArgs<float>& a, int np);
 int launch_grp_fwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
 int launch_grp_bwd_rerun(ChunkArgs<float> a, int np, hipStream_t st);
 // Adjoint zero-state responses only, for a ragged backward handed the forward's
-// summaries (ChunkArgs::kept): float32 orders 16 and 32.
+// summaries (ChunkArgs::kept): float32 orders 8, 16 and 32.
 bool grp_wsum_ok(const ChunkArgs<float>& a, int np);
 int launch_grp_wsum(ChunkArgs<float> a, int np, hipStream_t st);
 // The summary work item of a chunk's lanes in the multi-lane float32 summary

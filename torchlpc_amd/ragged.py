@@ -8,7 +8,7 @@ The reference filters rectangular ``[B, T]`` batches with one order
 * ``zi``: ``(sum N_k,)``, signal k's initial state as in ``sample_wise_lpc``.
 
 Each output equals ``sample_wise_lpc`` on the signal alone. With gradients on,
-the float32 order-16 and order-32 groups keep their forward's workspace (saved
+the float32 order-8, order-16 and order-32 groups keep their forward's workspace (saved
 for backward) and their backward reads its chunk transition matrices instead of
 forming them again. A :class:`RaggedPlan`
 holds the metadata (lengths, orders, offsets, and per order a device table that
@@ -207,10 +207,10 @@ _kept = threading.local()
 
 def keeps_summaries(grp, dtype: torch.dtype) -> bool:
     """Whether an order group's forward keeps its workspace for the backward:
-    float32 orders 16 and 32, whose backward then reads the forward's chunk
+    float32 orders 8, 16 and 32, whose backward then reads the forward's chunk
     transition matrices instead of forming its own (the library ignores a kept
     workspace elsewhere)."""
-    return dtype == torch.float32 and grp[0] in (16, 32)
+    return dtype == torch.float32 and grp[0] in (8, 16, 32)
 
 
 class RaggedLPC(Function):
