@@ -347,6 +347,10 @@ __device__ __forceinline__ void put(float4& o, int ql, float v) {  // sample K o
 }
 
 constexpr int RING = 4;  // A rows in flight per quad lane (LDS reads ahead of their step)
+// Keeps a ring refill where the step issues it: without it the scheduler sank
+// the LDS reads to a step or two before their use (and lent the ring's
+// registers to temporaries), so the steps waited for their rows.
+__device__ __forceinline__ void ring_pin() { __builtin_amdgcn_sched_barrier(0); }
 
 // Forward quad step at time t (slot J = t mod 4): lane ql's window R holds y_u
 // for u in [t-4-4ql, t-1-4ql] at slot u mod 4, c = A[t, 4ql .. 4ql+3]. The
@@ -373,6 +377,7 @@ __device__ __forceinline__ void fwd_steps(double (&R)[4], float4 (&cq)[RING], fl
   if constexpr (JJ < 16) {
     const double y = fwd_step<JJ & 3>(R, cq[JJ % RING], qsel<JJ>(xc), ql);
     cq[JJ % RING] = ar[min(tb + JJ + RING, TS - 1) * 4];
+    ring_pin();
     put<JJ>(out, ql, (float)y);
     fwd_steps<JJ + 1, TS>(R, cq, out, xc, ar, tb, ql);
   }
@@ -429,6 +434,7 @@ __device__ __forceinline__ void zs_steps(double (&P)[4], double& now, float4 (&c
     const int r = tb + 15 - JJ;
     adj_quad_step<J>(P, now, cq[JJ % RING], qsel<15 - JJ>(gc), ql, true);
     cq[JJ % RING] = ar[max(r - RING, 0) * 4];
+    ring_pin();
     zs_steps<JJ + 1>(P, now, cq, gc, ar, tb, ql);
   }
 }
@@ -442,7 +448,9 @@ __device__ __forceinline__ void zero_state(const BwdTile& L, double (&P)[4], dou
 #pragma unroll 1
   for (int jb = LC - 16; jb >= 0; jb -= 16) {
     const int tb = q * LC + jb;
-    zs_steps<0>(P, now, cq, L.g[tb / 4 + ql], ar, tb, ql);
+    float4 gc = L.g[tb / 4 + ql];
+    consume(gc);  // in registers for the block, not read from LDS again at every step
+    zs_steps<0>(P, now, cq, gc, ar, tb, ql);
   }
 }
 
@@ -477,6 +485,7 @@ __device__ __forceinline__ void lam_steps(V (&P)[4], V& now, float4 (&cq)[RING],
     const int r = tb + 15 - JJ;
     const V lam = adj_step<J>(P, now, cq[JJ % RING], qsel<15 - JJ>(gc), ql);
     cq[JJ % RING] = ar[max(r - RING, 0) * 4];
+    ring_pin();
     put<15 - JJ>(lo, ql, (float)lam);
     lam_steps<JJ + 1>(P, now, cq, lo, gc, ar, tb, ql);
   }
@@ -494,7 +503,9 @@ __device__ __forceinline__ void lam_rerun(const BwdTile& L, float4* lam4, V (&P)
   for (int jb = LC - 16; jb >= 0; jb -= 16) {
     const int tb = q * LC + jb;
     float4 lo = zero4();
-    lam_steps<0>(P, now, cq, lo, L.g[tb / 4 + ql], ar, tb, ql);
+    float4 gc = L.g[tb / 4 + ql];
+    consume(gc);  // in registers for the block, not read from LDS again at every step
+    lam_steps<0>(P, now, cq, lo, gc, ar, tb, ql);
     lam4[tb / 4 + ql] = lo;
   }
 }
