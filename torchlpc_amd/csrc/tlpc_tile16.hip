@@ -610,7 +610,9 @@ __global__ __launch_bounds__(THREADS) void tile_fwd(ChunkArgs<float> a) {
       phi_sweep<true>(L.A[bb], reinterpret_cast<const float*>(L.x[bb]), c, kk, ph, z);
       phi_store<true>(L.summ[p & 1][c], kk, ph, z);
     } else if (wave == FWD_CHAIN_WAVE && p >= 1 && p <= ntiles) {
-      chain<true, FC>(s, L.summ[(p - 1) & 1], L.carry[(p - 1) & 1], lane);
+      // on lanes 0-15 only: the other rows would repeat the chain and quadruple
+      // its LDS reads of the link matrices
+      if (lane < NP) chain<true, FC>(s, L.summ[(p - 1) & 1], L.carry[(p - 1) & 1], lane);
     } else if (wave == FWD_RERUN_WAVE && p >= 2) {
       const int64_t k = p - 2;
       const int bb = (int)(k % FNBUF);
@@ -672,7 +674,7 @@ __global__ __launch_bounds__(THREADS) void tile_bwd(ChunkArgs<float> a) {
     if (wave < 4) {
       if (p < ntiles) phi_sweep<false>(L.A[p % NBUF], nullptr, c, kk, ph, z);
     } else if (wave == CHAIN_WAVE) {
-      if (p >= 1 && p <= ntiles) chain<false>(s, L.summ, L.carry[(p - 1) & 1], lane);
+      if (p >= 1 && p <= ntiles && lane < NP) chain<false>(s, L.summ, L.carry[(p - 1) & 1], lane);
     } else if (wave == DA_WAVE) {
       if (p >= 3) {
         const int64_t k = p - 3;
