@@ -222,8 +222,19 @@ Plan make_plan(int dtype, int64_t B, int64_t T, int32_t n, int32_t flags, int32_
     p.L = pick_chunk_ti(B, T, p.np, chunk_len);
   else if (chunk16_ok(dtype, n, flags, neg, 256, T))
     p.L = pick_chunk16(B, T, chunk_len);
-  else
-    p.L = pick_chunk(B, T, p.np, chunk_len, multi_lane_rerun(dtype, n, p.np));
+  else {
+    const bool ml = multi_lane_rerun(dtype, n, p.np);
+    p.L = pick_chunk(B, T, p.np, chunk_len, ml);
+    // A small uniform batch of long signals would get many short chunks and
+    // long carry chains: at least min(64, T / 128) samples per chunk there
+    // (B=16, T=8192, N=4 forward: 512 chunks of 16 samples 29.6 us, 128 of 64
+    // samples 25.1 us, one HIP graph each)
+    if (ml && chunk_len <= 0) {
+      const int blk = chunk_block(p.np);
+      const int64_t lmin = std::min<int64_t>(64, T / 128);
+      if (p.L < lmin) p.L = (int)((lmin + blk - 1) / blk * blk);
+    }
+  }
   p.P = T > 0 ? (T + p.L - 1) / p.L : 0;
   const size_t ws_es = wide_size(dtype);
   // time-invariant summaries are the zero-state vectors only (Phi is per signal)
