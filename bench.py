@@ -793,7 +793,7 @@ def main():
         sec = {}
         for name in ("b16-t8192-n4", "b4-t1048576-n2", "ragged512", "ti-b1024-t16384-n32"):
             log(f"[bench] secondary workload {name}")
-            r = bench_workload(name, 10, 3, tune=not args.no_tune)
+            r = bench_workload(name, 30, 5, tune=not args.no_tune)
             sec[name] = {
                 "workload": r["config"]["workload"],
                 **({"chunk_len_by_order": r["config"]["chunk_len_by_order"]} if "chunk_len_by_order" in r["config"] else {}),
