@@ -559,11 +559,17 @@ __device__ __forceinline__ void grads(const float* lam, const float* ywin, float
 // Roles in period p: the Phi waves stage 1 (tile p), the chain wave stage 2
 // (tile p-1), the rerun wave stage 3 (tile p-2), and in the backward wave
 // DA_WAVE stage 4 (tile p-3); every wave stages tile p+1. Wave w runs on SIMD
-// w % 4. Tile p
+// w % 4: the forward's three Phi waves take SIMDs 0-2 and its rerun and chain
+// SIMD 3; the backward's four Phi waves share SIMDs 0 and 1, its zero-state and
+// lambda quads SIMD 2, its chain and dA stage SIMD 3. Tile p
 // lives in buffer p % NBUF (the backward's y window in p % (NBUF + 1)), its
 // carries in carry[p % 2].
-constexpr int ZS_WAVE = 4;       // backward: zero-state responses (stage 1)
-constexpr int CHAIN_WAVE = 5;
+// backward: Phi sweeps (stage 1) on waves 0, 1, 4, 5, two on each of SIMDs 0
+// and 1; the serial stages take SIMDs 2 and 3
+constexpr bool bwd_phi(int w) { return (w & 2) == 0; }
+constexpr int bwd_phi_index(int w) { return (w & 1) | ((w >> 2) << 1); }
+constexpr int ZS_WAVE = 2;       // backward: zero-state responses (stage 1)
+constexpr int CHAIN_WAVE = 3;
 constexpr int FWD_PHI_WAVES = FC / 4;  // forward: Phi sweeps (stage 1), waves 0-2
 constexpr int FWD_RERUN_WAVE = 3;      // forward: rerun and y stores (stage 3), SIMD 3
 constexpr int FWD_CHAIN_WAVE = 7;      // forward: carry chain (stage 2), SIMD 3
@@ -667,11 +673,11 @@ __global__ __launch_bounds__(THREADS) void tile_bwd(ChunkArgs<float> a) {
     st.load_rows(Ab, T, max(ntiles - 2 - p, (int64_t)0) * S, si);
     st.load_gy(gb, yb, zib, T, si);
     double ph[NP], z = 0.0;
-    const int kk = lane & 15, c = wave * 4 + (lane >> 4);
+    const int kk = lane & 15, c = bwd_phi_index(wave) * 4 + (lane >> 4);
     const int q = lane >> 2, ql = lane & 3;
     double P[4] = {0.0, 0.0, 0.0, 0.0};
     double now = 0.0;
-    if (wave < 4) {
+    if (bwd_phi(wave)) {
       if (p < ntiles) phi_sweep<false>(L.A[p % NBUF], nullptr, c, kk, ph, z);
     } else if (wave == CHAIN_WAVE) {
       if (p >= 1 && p <= ntiles && lane < NP) chain<false>(s, L.summ, L.carry[(p - 1) & 1], lane);
@@ -711,7 +717,7 @@ __global__ __launch_bounds__(THREADS) void tile_bwd(ChunkArgs<float> a) {
       }
     }
     lds_barrier();
-    if (wave < 4 && p < ntiles) phi_store<false>(L.summ[c], kk, ph, z);
+    if (bwd_phi(wave) && p < ntiles) phi_store<false>(L.summ[c], kk, ph, z);
     if (wave == ZS_WAVE && p < ntiles) {
       double* w = &L.summ[q][NP * ROWP];
       if (ql == 0) w[0] = now;
